@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Boid-step throughput on one MI355X: FP64 plain vs mixed-precision grid path.
+
+Workload (default): N = 2^20 boids, cubic field F = 4.0 (density 16384 per
+unit volume), uniform positions, isotropic directions with |v| ~ U[0.001,
+0.005] (seed 2), rules: radii 0.02/0.05/0.08 (separation/alignment/
+cohesion), view angle 270 deg, weights 0.002/0.06/0.008, dt = 1, periodic
+wrap.  A "step" is one full update() of all N boids: fused neighbour +
+integrate kernel, then the counting sort that re-bins the new positions.
+
+Prints ONE JSON line.  `value` is the mixed-precision path's boid-steps/s
+with the state resident in HBM; the FP64 plain figure is reported beside it.
+`roofline` is for the dominant kernel (fused neighbour + integrate) of the
+headline path, from HIP events on the engine's stream; `cpu_baseline` times
+the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import boidssimulatormpi_amd as B  # noqa: E402
+from boidssimulatormpi_amd import engine as E  # noqa: E402
+
+RULES = dict(sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008), vmax=0.005, vmin=0.001)
+BYTES_PER_BOID_STEP = 96          # read + write of the FP64 master state (pos + vel, 6 x 8 B)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_mode(mode, init, n, field, steps, warmup):
+    e = B.Engine(B.make_params(n, field, **RULES, mode=mode, flags=E.FLAG_TIMING))
+    info = e.info()
+    e.set_state(xyzv6=init)
+    e.step(warmup)
+    e.sync()
+    e.reset_timing()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    t1 = time.perf_counter()
+    kernels = {}
+    for k, name in enumerate(E.K_NAMES):
+        ms, cnt = e.timing(k)
+        if cnt:
+            kernels[name] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
+    # physical sanity of the end state (cheap, outside the timed region)
+    p, v = e.state_f64()
+    s = np.linalg.norm(v, axis=1)
+    ok = bool(np.isfinite(p).all() and (s <= RULES["vmax"] * (1 + 1e-9)).all())
+    rechecks = e.rechecks()
+    e.close()
+    wall = t1 - t0
+    return {"mode": E.MODE_NAMES[mode], "cells_per_axis": info.cells_per_axis,
+            "ms_per_step": 1e3 * wall / steps, "boid_steps_per_s": n * steps / wall,
+            "kernels": kernels, "sane": ok, "fp64_rechecks": rechecks}
+
+
+def cpu_baseline(init, n, field, budget_s):
+    """Oracle FP64 cell-list step (the reference algorithm restated, OpenMP)
+    on the same workload; steps until ~budget_s of wall time."""
+    from oracle import oracle as O
+    P = O.params(field, RULES["sep"], RULES["ali"], RULES["coh"], RULES["vmax"], RULES["vmin"])
+    p = init[:, :3].astype(np.float64).copy()
+    v = init[:, 3:].astype(np.float64).copy()
+    threads = len(os.sched_getaffinity(0))
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        O.step(P, p, v, enumeration=O.ENUM_CELLS, nthreads=threads)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    wall = time.perf_counter() - t0
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return {"value": n * steps / wall, "unit": "boid-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} full step(s) of the N={n} workload, oracle FP64 cell list, "
+                      f"{threads} OpenMP threads, {wall:.1f} s wall; CPU: {model}"}
+
+
+def traffic_from_profiles(mode_name, n):
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_pair_{mode_name}_n{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--field", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--modes", default="grid_f64,grid_mixed")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the path's "
+              "multi-GPU sharding is a design (spatial slabs with halo exchange), not a runnable mode.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n, F = a.n, a.field
+    init = B.generate_init(E.INIT_ISOTROPIC, n, F, RULES["vmax"], RULES["vmin"], seed=a.seed)
+    by_name = {v: k for k, v in E.MODE_NAMES.items()}
+    res = {}
+    for name in a.modes.split(","):
+        log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} steps={a.steps}")
+        res[name] = run_mode(by_name[name], init, n, F, a.steps, a.warmup)
+        log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
+
+    head = res.get("grid_mixed") or next(iter(res.values()))
+    pair = head["kernels"].get("pair", {"us_avg": float("nan")})
+    achieved = BYTES_PER_BOID_STEP * n / (pair["us_avg"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(head["mode"], n),
+                "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": BYTES_PER_BOID_STEP}
+    cpu = None
+    if not a.no_cpu:
+        log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(init, n, F, a.cpu_seconds)
+    out = {
+        "metric": "boid-steps/sec at N=2^20: FP64 plain vs mixed precision; % HBM roofline",
+        "value": head["boid_steps_per_s"],
+        "unit": "boid-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head["mode"] == "grid_mixed" else "f64",
+        "data": "synthetic (isotropic uniform init, seed %d)" % a.seed,
+        "config": {"workload": f"grid boid step, N={n}, F={F}, uniform positions, isotropic velocities "
+                               f"|v| in [0.001,0.005], radii 0.02/0.05/0.08, view 270 deg, weights "
+                               f"0.002/0.06/0.008, seed {a.seed}",
+                   "n": n, "field": F, "headline_mode": head["mode"], "parallelism": "single GPU"},
+        "fp64_plain": res.get("grid_f64"),
+        "mixed": res.get("grid_mixed"),
+        "mixed_over_fp64": (res["grid_mixed"]["boid_steps_per_s"] / res["grid_f64"]["boid_steps_per_s"]
+                            if "grid_mixed" in res and "grid_f64" in res else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

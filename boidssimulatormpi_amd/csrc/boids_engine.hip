@@ -1,0 +1,831 @@
+// boids_engine.hip — host side of libboids_engine.so: the C ABI declared in
+// include/boids_engine.h, exact-threshold derivation, init generators and
+// the per-step launch sequence.  Kernels live in boids_kernels.hip.
+#include "boids_engine.h"
+#include "boids_internal.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace boids;
+
+static thread_local std::string g_create_error;
+
+struct bs_engine {
+    bs_params prm{};
+    int mode = 0;          // effective mode
+    int G = 0;
+    double edge = 0.0;
+    DevParams P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool has_state = false;
+    uint64_t steps = 0;
+    uint64_t device_bytes = 0;
+    std::string err;
+    std::vector<void*> allocs;
+
+    // state buffers (A = current, B = next)
+    StateF32 fa{}, fb{};
+    StateF64 da{}, db{};
+    // grid
+    uint32_t *key = nullptr, *key_sorted = nullptr, *count = nullptr, *start = nullptr;
+    uint32_t *partials = nullptr, *nchunks = nullptr, *perm = nullptr, *permid = nullptr;
+    uint4* table = nullptr;
+    uint32_t max_chunks = 0, ncell1 = 0;
+    Lattice lat{};
+    uint32_t* recheck = nullptr;
+    // outputs
+    void* forces = nullptr;   // float or double, 9n, id order
+    uint64_t* digest = nullptr;
+    double *tmp_pos = nullptr, *tmp_vel = nullptr;
+
+    // timing
+    struct Pending { int kernel; hipEvent_t a, b; };
+    std::vector<hipEvent_t> free_events;
+    std::vector<Pending> pending;
+    double ms[BS_K_COUNT] = {0};
+    uint64_t launches[BS_K_COUNT] = {0};
+};
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+static int fail(bs_engine* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg; else g_create_error = msg;
+    return code;
+}
+
+static int hip_fail(bs_engine* e, hipError_t st, const char* what)
+{
+    return fail(e, BS_E_HIP, std::string(what) + ": " + hipGetErrorString(st));
+}
+
+#define HIP_TRY(e, expr)                                               \
+    do {                                                               \
+        hipError_t st_ = (expr);                                       \
+        if (st_ != hipSuccess) return hip_fail((e), st_, #expr);       \
+    } while (0)
+
+template <typename T>
+static int dev_alloc(bs_engine* e, T** p, size_t count)
+{
+    size_t bytes = std::max<size_t>(1, count * sizeof(T));
+    void* ptr = nullptr;
+    hipError_t st = hipMalloc(&ptr, bytes);
+    if (st != hipSuccess) return fail(e, BS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    e->allocs.push_back(ptr);
+    e->device_bytes += bytes;
+    *p = (T*)ptr;
+    return BS_OK;
+}
+
+static hipEvent_t take_event(bs_engine* e)
+{
+    if (!e->free_events.empty()) {
+        hipEvent_t ev = e->free_events.back();
+        e->free_events.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Brackets one launch group with HIP events on the engine's stream.
+struct Timed {
+    bs_engine* e;
+    int kernel;
+    hipEvent_t a = nullptr;
+    Timed(bs_engine* e_, int k) : e(e_), kernel(k)
+    {
+        if (e->prm.flags & BS_FLAG_TIMING) {
+            a = take_event(e);
+            if (a) (void)hipEventRecord(a, e->stream);
+        }
+    }
+    void stop()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(e);
+        if (!b) { e->free_events.push_back(a); a = nullptr; return; }
+        (void)hipEventRecord(b, e->stream);
+        e->pending.push_back({kernel, a, b});
+        a = nullptr;
+    }
+};
+
+static int resolve_timing(bs_engine* e)
+{
+    if (e->pending.empty()) return BS_OK;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    for (auto& p : e->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            e->ms[p.kernel] += ms;
+            e->launches[p.kernel] += 1;
+        }
+        e->free_events.push_back(p.a);
+        e->free_events.push_back(p.b);
+    }
+    e->pending.clear();
+    return BS_OK;
+}
+
+// ----------------------------------------------------------------------------
+// exact thresholds: the kernels decide `sqrt(s) < r` and `acos(c) <= pi*a`
+// by comparing s and c with the boundary values below.  The rounded sqrt is
+// monotone, so {s : sqrt(s) < r} = [0, s_thr).  The host acos is checked to be
+// monotone around the boundary, so {c : acos(c) <= pi*a} = [c_thr, 1].
+// ----------------------------------------------------------------------------
+static double sthr_f64(double r)
+{
+    if (isnan(r)) return -1.0;
+    if (!(r > 0.0)) return 0.0;
+    if (isinf(r)) return INFINITY;
+    double s = r * r;
+    if (isinf(s)) s = DBL_MAX;
+    while (s > 0.0 && sqrt(s) >= r) s = nextafter(s, 0.0);
+    while (!(sqrt(s) >= r)) {
+        if (isinf(s)) break;
+        s = nextafter(s, INFINITY);
+    }
+    return s;
+}
+
+static float sthr_f32(double r)
+{
+    if (isnan(r)) return -1.0f;
+    if (!(r > 0.0)) return 0.0f;
+    float s = (float)(r * r);
+    if (isinf(s)) s = FLT_MAX;
+    while (s > 0.0f && (double)sqrtf(s) >= r) s = nextafterf(s, 0.0f);
+    while (!((double)sqrtf(s) >= r)) {
+        if (isinf(s)) break;
+        s = nextafterf(s, INFINITY);
+    }
+    return s;
+}
+
+// ordinal <-> float over [-1, 1]
+static int64_t ord32(float c) { uint32_t u; memcpy(&u, &c, 4); return (u & 0x80000000u) ? -(int64_t)(u & 0x7fffffffu) : (int64_t)u; }
+static float from_ord32(int64_t o)
+{
+    uint32_t u = o < 0 ? (0x80000000u | (uint32_t)(-o)) : (uint32_t)o;
+    float c; memcpy(&c, &u, 4); return c;
+}
+static int64_t ord64(double c) { uint64_t u; memcpy(&u, &c, 8); return (u >> 63) ? -(int64_t)(u & 0x7fffffffffffffffull) : (int64_t)u; }
+static double from_ord64(int64_t o)
+{
+    uint64_t u = o < 0 ? (0x8000000000000000ull | (uint64_t)(-o)) : (uint64_t)o;
+    double c; memcpy(&c, &u, 8); return c;
+}
+
+// Smallest c in [-1,1] with acos(c) <= theta (ACOS = acosf on float, acos on
+// double).  Returns 2.0 when none.  *ok = 0 if the predicate is not a clean
+// false..true step within +-window ordinals of the boundary.
+template <typename T, typename FN>
+static double cone_threshold(double theta, FN acos_fn, int64_t (*ord)(T), T (*from)(int64_t),
+                             int64_t window, int* ok)
+{
+    auto pred = [&](int64_t o) { return (double)acos_fn(from(o)) <= theta; };
+    const int64_t lo0 = ord((T)-1.0), hi0 = ord((T)1.0);
+    *ok = 1;
+    if (!pred(hi0)) return 2.0;
+    if (pred(lo0)) return -1.0;
+    int64_t lo = lo0, hi = hi0;      // pred(lo) false, pred(hi) true
+    while (hi - lo > 1) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (pred(mid)) hi = mid; else lo = mid;
+    }
+    for (int64_t o = std::max(lo0, hi - window); o < hi; ++o)
+        if (pred(o)) { *ok = 0; break; }
+    for (int64_t o = hi; o <= std::min(hi0, hi + window); ++o)
+        if (!pred(o)) { *ok = 0; break; }
+    return (double)from(hi);
+}
+
+static float round_down_f(double x) { float f = (float)x; if ((double)f > x) f = nextafterf(f, -INFINITY); return f; }
+static float round_up_f(double x) { float f = (float)x; if ((double)f < x) f = nextafterf(f, INFINITY); return f; }
+
+static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
+{
+    const double r[3] = {p->sep_r, p->ali_r, p->coh_r};
+    const double a[3] = {p->sep_a, p->ali_a, p->coh_a};
+    const double w[3] = {p->sep_w, p->ali_w, p->coh_w};
+    P->F = p->field;
+    P->half = p->field / 2;
+    P->s_max = -1.0;
+    P->s_max_f = -1.0f;
+    P->unit = p->field / 4294967296.0;
+    P->lat_scale = 4294967296.0 / p->field;
+    P->s_hi_max = 0.0f;
+    const double beta_c = 1e-5;
+    for (int k = 0; k < 3; ++k) {
+        P->s_thr[k] = sthr_f64(r[k]);
+        P->s_thr_f[k] = sthr_f32(r[k]);
+        P->s_max = std::max(P->s_max, P->s_thr[k]);
+        P->s_max_f = std::max(P->s_max_f, P->s_thr_f[k]);
+        P->w[k] = w[k];
+        P->w_f[k] = (float)w[k];
+        P->cone[k] = !(a[k] >= 1.0);   // src/boid.cpp:29: a >= 1 skips the cone (NaN keeps it)
+        P->never[k] = 0;
+        P->c_thr[k] = -2.0;
+        P->c_thr_f[k] = -2.0f;
+        if (P->cone[k]) {
+            const double theta = M_PI * a[k];
+            int ok32 = 1, ok64 = 1;
+            double c32 = cone_threshold<float>(theta, [](float c) { return acosf(c); }, ord32, from_ord32, 1 << 16, &ok32);
+            double c64 = cone_threshold<double>(theta, [](double c) { return acos(c); }, ord64, from_ord64, 1 << 14, &ok64);
+            if (!ok32 || !ok64) {
+                *why = "host acos/acosf is not monotone around the view-cone boundary of rule " + std::to_string(k);
+                return BS_E_MATH;
+            }
+            P->c_thr_f[k] = (float)c32;
+            P->c_thr[k] = c64;
+            P->never[k] = c64 > 1.0;
+        }
+        // mixed guard bands (lattice units); beta_d covers the FP32 distance
+        // error (relative ~3e-7) and the lattice rounding (sqrt(3) units).
+        const double beta_d = 1e-5 + 2e-6 * (isfinite(r[k]) ? fabs(r[k]) : 0.0);
+        const double lo = r[k] - beta_d, hi = r[k] + beta_d;
+        P->s_lo[k] = (lo > 0.0) ? round_down_f((lo / P->unit) * (lo / P->unit)) : 0.0f;
+        P->s_hi[k] = (hi > 0.0) ? round_up_f((hi / P->unit) * (hi / P->unit)) : 0.0f;
+        if (isnan(r[k])) { P->s_lo[k] = 0.0f; P->s_hi[k] = 0.0f; }
+        P->s_hi_max = std::max(P->s_hi_max, P->s_hi[k]);
+        const double cth = P->c_thr[k];
+        P->c_lo[k] = round_down_f(cth - beta_c);
+        P->c_hi[k] = round_up_f(cth + beta_c);
+    }
+    P->c_top = round_down_f(1.0 - beta_c);
+    P->band_k = 8.0f;   // >= 2*sqrt(3) lattice units of displacement error, x2 margin
+    P->vmax = p->vmax;
+    P->vmin = p->vmin;
+    P->vmax_f = (float)p->vmax;
+    P->vmin_f = (float)p->vmin;
+    return BS_OK;
+}
+
+static int cells_per_axis(const bs_params* p)
+{
+    if (p->cells_per_axis > 0) {
+        double rmax = std::max(p->coh_r, std::max(p->ali_r, p->sep_r));
+        double edge = p->field / p->cells_per_axis;
+        if (p->cells_per_axis < 3 || !(edge >= rmax * 1.0001)) return -1;
+        return std::min(p->cells_per_axis, kMaxCells);
+    }
+    double rmax = std::max(p->coh_r, std::max(p->ali_r, p->sep_r));
+    if (!(rmax > 0.0) || !(p->field > 0.0)) return 0;
+    double g = floor(p->field / (rmax * 1.0001));
+    if (g < 3.0) return 0;
+    // no more cells than boids: finer grids only add empty-cell overhead
+    double cap = std::max(3.0, floor(cbrt((double)std::max<uint32_t>(p->n, 27))));
+    g = std::min(g, std::max(cap, 3.0));
+    g = std::min(g, (double)kMaxCells);
+    return (int)g;
+}
+
+// ----------------------------------------------------------------------------
+// init generators (host): drand48 restated, reference kinds + workloads
+// ----------------------------------------------------------------------------
+namespace {
+struct Drand48 {
+    uint64_t x = 0;                       // glibc's state before any srand48
+    void seed(long s) { x = (((uint64_t)(uint32_t)s) << 16) | 0x330Eu; }
+    double next()
+    {
+        x = (0x5DEECE66Dull * x + 0xBull) & ((1ull << 48) - 1);
+        return (double)x * (1.0 / 281474976710656.0);
+    }
+};
+
+double gauss(Drand48& r)
+{
+    const double u1 = 1.0 - r.next();     // (0, 1]
+    const double u2 = r.next();
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+double wrap_field(double x, double F)
+{
+    x -= F * floor(x / F);
+    if (x >= F) x -= F;
+    if (x < 0.0) x = 0.0;
+    return x;
+}
+}  // namespace
+
+extern "C" int bs_generate_init(int32_t kind, int32_t seeded, int32_t seed, uint32_t n, double F,
+                                double vmax, double vmin, float* out)
+{
+    if ((n && !out) || kind < 0 || kind > 3) return fail(nullptr, BS_E_ARG, "bs_generate_init: bad arguments");
+    Drand48 r;
+    if (seeded) r.seed(seed);
+    if (kind == BS_INIT_TEST || kind == BS_INIT_RANDOM_UNIFORM) {
+        // src/boid_simulation.cpp:82-112, draw for draw
+        for (uint32_t i = 0; i < n; ++i) {
+            float* o = out + 6 * (size_t)i;
+            if (kind == BS_INIT_TEST) {
+                o[0] = (float)(F * 3 / 8 + r.next() * F / 4);
+                o[1] = (float)(F * 3 / 8 + r.next() * F / 4);
+                if (r.next() > 0.5) o[2] = (float)(r.next() * F / 4 + F * 3 / 4);
+                else o[2] = (float)(r.next() * F / 4);
+            } else {
+                o[0] = (float)(F * r.next());
+                o[1] = (float)(F * r.next());
+                o[2] = (float)(F * r.next());
+            }
+            double s = r.next() * (vmax - vmin) + vmin;
+            double th1 = r.next() * M_PI;
+            double th2 = r.next() * 2.0 * M_PI;
+            o[3] = (float)(s * sin(th1) * cos(th2));
+            o[4] = (float)(s * sin(th1) * sin(th2));
+            o[5] = (float)(s * cos(th1));
+        }
+        return BS_OK;
+    }
+    if (kind == BS_INIT_ISOTROPIC) {
+        // per boid: x, y, z, speed, cos(theta) = 2u-1, phi = 2 pi u
+        for (uint32_t i = 0; i < n; ++i) {
+            float* o = out + 6 * (size_t)i;
+            o[0] = (float)(F * r.next());
+            o[1] = (float)(F * r.next());
+            o[2] = (float)(F * r.next());
+            double s = vmin + (vmax - vmin) * r.next();
+            double ct = 2.0 * r.next() - 1.0;
+            double ph = 2.0 * M_PI * r.next();
+            double st = sqrt(std::max(0.0, 1.0 - ct * ct));
+            o[3] = (float)(s * st * cos(ph));
+            o[4] = (float)(s * st * sin(ph));
+            o[5] = (float)(s * ct);
+        }
+        return BS_OK;
+    }
+    // BS_INIT_CLUSTERED: 64 blobs, centres uniform, sigma = 0.05 F, positions
+    // wrapped; heading per blob isotropic, boid direction = heading rotated
+    // by N(0, 0.2) rad about a uniformly random perpendicular axis.
+    const int nb = 64;
+    double cen[64][3], hd[64][3];
+    for (int b = 0; b < nb; ++b) {
+        for (int k = 0; k < 3; ++k) cen[b][k] = F * r.next();
+        double ct = 2.0 * r.next() - 1.0, ph = 2.0 * M_PI * r.next();
+        double st = sqrt(std::max(0.0, 1.0 - ct * ct));
+        hd[b][0] = st * cos(ph); hd[b][1] = st * sin(ph); hd[b][2] = ct;
+    }
+    const double sigma = 0.05 * F;
+    for (uint32_t i = 0; i < n; ++i) {
+        float* o = out + 6 * (size_t)i;
+        int b = (int)(nb * r.next());
+        if (b >= nb) b = nb - 1;
+        for (int k = 0; k < 3; ++k) o[k] = (float)wrap_field(cen[b][k] + sigma * gauss(r), F);
+        const double* h = hd[b];
+        double ax[3] = {1.0, 0.0, 0.0};
+        if (fabs(h[0]) > 0.9) { ax[0] = 0.0; ax[1] = 1.0; }
+        double e1[3] = {ax[1] * h[2] - ax[2] * h[1], ax[2] * h[0] - ax[0] * h[2], ax[0] * h[1] - ax[1] * h[0]};
+        double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        for (int k = 0; k < 3; ++k) e1[k] /= l1;
+        double e2[3] = {h[1] * e1[2] - h[2] * e1[1], h[2] * e1[0] - h[0] * e1[2], h[0] * e1[1] - h[1] * e1[0]};
+        double th = 0.2 * gauss(r), ph = 2.0 * M_PI * r.next();
+        double s = vmin + (vmax - vmin) * r.next();
+        for (int k = 0; k < 3; ++k) {
+            double d = cos(th) * h[k] + sin(th) * (cos(ph) * e1[k] + sin(ph) * e2[k]);
+            o[3 + k] = (float)(s * d);
+        }
+    }
+    return BS_OK;
+}
+
+extern "C" int bs_read_init_file(const char* path, uint32_t n, float* out)
+{
+    if (!path || (n && !out)) return fail(nullptr, BS_E_ARG, "bs_read_init_file: bad arguments");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(nullptr, BS_E_ARG, std::string("Invalid initial placement type or file name: ") + path);
+    f.read((char*)out, (std::streamsize)(sizeof(float) * 6 * (size_t)n));
+    if ((size_t)f.gcount() != sizeof(float) * 6 * (size_t)n)
+        return fail(nullptr, BS_E_ARG, std::string("init file too short: ") + path);
+    return BS_OK;   // little-endian host: the file layout is the memory layout
+}
+
+// ----------------------------------------------------------------------------
+// engine lifetime
+// ----------------------------------------------------------------------------
+static void release(bs_engine* e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto& p : e->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto ev : e->free_events) (void)hipEventDestroy(ev);
+    for (void* p : e->allocs) (void)hipFree(p);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static int alloc_f64(bs_engine* e, StateF64* s, uint32_t n)
+{
+    int rc;
+    if ((rc = dev_alloc(e, &s->x, n)) || (rc = dev_alloc(e, &s->y, n)) || (rc = dev_alloc(e, &s->z, n)) ||
+        (rc = dev_alloc(e, &s->vx, n)) || (rc = dev_alloc(e, &s->vy, n)) || (rc = dev_alloc(e, &s->vz, n)) ||
+        (rc = dev_alloc(e, &s->id, n)))
+        return rc;
+    return BS_OK;
+}
+
+static int alloc_f32(bs_engine* e, StateF32* s, uint32_t n)
+{
+    int rc;
+    if ((rc = dev_alloc(e, &s->x, n)) || (rc = dev_alloc(e, &s->y, n)) || (rc = dev_alloc(e, &s->z, n)) ||
+        (rc = dev_alloc(e, &s->vx, n)) || (rc = dev_alloc(e, &s->vy, n)) || (rc = dev_alloc(e, &s->vz, n)))
+        return rc;
+    return BS_OK;
+}
+
+extern "C" int bs_create(const bs_params* p, bs_engine** out)
+{
+    if (!out) return fail(nullptr, BS_E_ARG, "bs_create: out is NULL");
+    *out = nullptr;
+    if (!p) return fail(nullptr, BS_E_ARG, "bs_create: params is NULL");
+    if (!(p->field > 0.0) || !isfinite(p->field)) return fail(nullptr, BS_E_ARG, "field size must be positive and finite");
+    if (p->mode < BS_MODE_ALLPAIRS_F32REF || p->mode > BS_MODE_GRID_MIXED) return fail(nullptr, BS_E_ARG, "unknown mode");
+    if (p->n > (1u << 28)) return fail(nullptr, BS_E_ARG, "n too large (max 2^28)");
+
+    bs_engine* e = new bs_engine();
+    e->prm = *p;
+    e->mode = p->mode;
+    std::string why;
+    int rc = derive_thresholds(p, &e->P, &why);
+    if (rc) { g_create_error = why; release(e); return rc; }
+    e->P.n = p->n;
+
+    if (e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED) {
+        int G = cells_per_axis(p);
+        if (G < 0) { release(e); return fail(nullptr, BS_E_ARG, "cells_per_axis must be >= 3 with edge >= max radius"); }
+        if (G < 3) {
+            e->mode = BS_MODE_ALLPAIRS_F64;   // fewer than 3 cells per axis: 27-cell windows would overlap
+        } else {
+            e->G = G;
+            e->edge = p->field / G;
+            e->P.G = G;
+            e->P.inv_edge = G / p->field;
+        }
+    }
+    if (e->mode == BS_MODE_GRID_MIXED && !(e->P.unit * 8.0 < 1e-6)) e->mode = BS_MODE_GRID_F64;   // lattice too coarse
+
+    hipError_t st = hipSetDevice(p->device);
+    if (st != hipSuccess) { std::string m = std::string("hipSetDevice: ") + hipGetErrorString(st); release(e); g_create_error = m; return BS_E_HIP; }
+    e->device = p->device;
+    st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (st != hipSuccess) { std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(st); e->stream = nullptr; release(e); g_create_error = m; return BS_E_HIP; }
+
+    const uint32_t n = p->n;
+    auto bail = [&](int code) { std::string m = e->err; release(e); g_create_error = m; return code; };
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        if ((rc = alloc_f32(e, &e->fa, n)) || (rc = alloc_f32(e, &e->fb, n))) return bail(rc);
+        if (p->flags & BS_FLAG_FORCES) { float* f; if ((rc = dev_alloc(e, &f, 9 * (size_t)n))) return bail(rc); e->forces = f; }
+    } else {
+        if ((rc = alloc_f64(e, &e->da, n)) || (rc = alloc_f64(e, &e->db, n))) return bail(rc);
+        if (e->mode == BS_MODE_ALLPAIRS_F64) e->db.id = e->da.id;   // all-pairs stays in id order
+        if (p->flags & BS_FLAG_FORCES) { double* f; if ((rc = dev_alloc(e, &f, 9 * (size_t)n))) return bail(rc); e->forces = f; }
+    }
+    if ((rc = dev_alloc(e, &e->tmp_pos, 3 * (size_t)n)) || (rc = dev_alloc(e, &e->tmp_vel, 3 * (size_t)n))) return bail(rc);
+    if (p->flags & BS_FLAG_DIGEST) { if ((rc = dev_alloc(e, &e->digest, 4 * (size_t)n))) return bail(rc); }
+    if (e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED) {
+        const uint64_t G = (uint64_t)e->G;
+        e->ncell1 = (uint32_t)(G * G * G + 1);
+        e->max_chunks = (n + kBlock - 1) / kBlock + (uint32_t)(G * G) + 1;
+        const uint32_t ntiles = (e->ncell1 + 4095) / 4096;
+        if ((rc = dev_alloc(e, &e->key, n)) || (rc = dev_alloc(e, &e->key_sorted, n)) ||
+            (rc = dev_alloc(e, &e->count, e->ncell1)) || (rc = dev_alloc(e, &e->start, e->ncell1)) ||
+            (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->nchunks, 1)) ||
+            (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) ||
+            (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)))
+            return bail(rc);
+        if (e->mode == BS_MODE_GRID_MIXED) {
+            if ((rc = dev_alloc(e, &e->lat.p, n)) || (rc = dev_alloc(e, &e->lat.vx, n)) ||
+                (rc = dev_alloc(e, &e->lat.vy, n)) || (rc = dev_alloc(e, &e->lat.vz, n)))
+                return bail(rc);
+        }
+        st = hipMemsetAsync(e->count, 0, sizeof(uint32_t) * e->ncell1, e->stream);
+        if (st == hipSuccess) st = hipMemsetAsync(e->recheck, 0, sizeof(uint32_t), e->stream);
+        if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
+    }
+    if (e->forces) {
+        size_t bytes = 9 * (size_t)n * (e->mode == BS_MODE_ALLPAIRS_F32REF ? sizeof(float) : sizeof(double));
+        st = hipMemsetAsync(e->forces, 0, bytes, e->stream);
+        if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
+    }
+    if (e->digest) {
+        st = hipMemsetAsync(e->digest, 0, 32 * (size_t)n, e->stream);
+        if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
+    }
+    st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) { hip_fail(e, st, "hipStreamSynchronize"); return bail(BS_E_HIP); }
+    *out = e;
+    return BS_OK;
+}
+
+extern "C" void bs_destroy(bs_engine* e) { release(e); }
+
+extern "C" const char* bs_last_error(const bs_engine* e)
+{
+    return e ? e->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" const char* bs_version(void) { return "boids_engine 0.1 (gfx950)"; }
+
+// ----------------------------------------------------------------------------
+// state in / out
+// ----------------------------------------------------------------------------
+// Sort the B-side (id order or previous-slot order) state into A by (cell, id).
+static int grid_sort_into_a(bs_engine* e)
+{
+    Timed t(e, BS_K_SCAN);
+    HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
+    HIP_TRY(e, launch_rows(e->P, e->start, e->table, e->nchunks, e->stream));
+    t.stop();
+    Timed t2(e, BS_K_SCATTER);
+    HIP_TRY(e, launch_scatter(e->P, e->key, e->start, e->count, e->db.id, e->perm, e->permid, e->stream));
+    t2.stop();
+    Timed t3(e, BS_K_GATHER);
+    HIP_TRY(e, launch_gather(e->P, e->perm, e->permid, e->key, e->start, e->count, e->db, e->da,
+                             e->key_sorted, e->mode == BS_MODE_GRID_MIXED ? &e->lat : nullptr, e->stream));
+    t3.stop();
+    return BS_OK;
+}
+
+static int upload_f64(bs_engine* e, const double* pos3, const double* vel3)
+{
+    const uint32_t n = e->prm.n;
+    std::vector<double> buf(7 * (size_t)n);
+    double *x = buf.data(), *y = x + n, *z = y + n, *vx = z + n, *vy = vx + n, *vz = vy + n;
+    uint32_t* ids = (uint32_t*)(vz + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        x[i] = pos3[3 * (size_t)i]; y[i] = pos3[3 * (size_t)i + 1]; z[i] = pos3[3 * (size_t)i + 2];
+        vx[i] = vel3[3 * (size_t)i]; vy[i] = vel3[3 * (size_t)i + 1]; vz[i] = vel3[3 * (size_t)i + 2];
+        ids[i] = i;
+    }
+    const bool grid = e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED;
+    StateF64 dst = grid ? e->db : e->da;
+    const size_t b = sizeof(double) * n;
+    HIP_TRY(e, hipMemcpyAsync(dst.x, x, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.y, y, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.z, z, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.vx, vx, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.vy, vy, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.vz, vz, b, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(dst.id, ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, e->stream));
+    if (grid) {
+        HIP_TRY(e, launch_hash_count(e->P, e->db, e->key, e->count, e->stream));
+        int rc = grid_sort_into_a(e);
+        if (rc) return rc;
+    }
+    HIP_TRY(e, hipStreamSynchronize(e->stream));   // the host staging buffer dies here
+    e->has_state = true;
+    e->steps = 0;
+    return BS_OK;
+}
+
+extern "C" int bs_set_state_f64(bs_engine* e, const double* pos3, const double* vel3)
+{
+    if (!e) return BS_E_ARG;
+    if (e->prm.n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_set_state_f64: NULL array");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) return fail(e, BS_E_STATE, "float-state mode: use bs_set_state_f32");
+    return upload_f64(e, pos3, vel3);
+}
+
+extern "C" int bs_set_state_f32(bs_engine* e, const float* xyzv6)
+{
+    if (!e) return BS_E_ARG;
+    const uint32_t n = e->prm.n;
+    if (n && !xyzv6) return fail(e, BS_E_ARG, "bs_set_state_f32: NULL array");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        std::vector<float> buf(6 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 6; ++k) buf[(size_t)k * n + i] = xyzv6[6 * (size_t)i + k];
+        float* dst[6] = {e->fa.x, e->fa.y, e->fa.z, e->fa.vx, e->fa.vy, e->fa.vz};
+        for (int k = 0; k < 6; ++k)
+            HIP_TRY(e, hipMemcpyAsync(dst[k], buf.data() + (size_t)k * n, sizeof(float) * n, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        e->has_state = true;
+        e->steps = 0;
+        return BS_OK;
+    }
+    std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * (size_t)i + k] = xyzv6[6 * (size_t)i + k];
+            vel[3 * (size_t)i + k] = xyzv6[6 * (size_t)i + 3 + k];
+        }
+    return upload_f64(e, pos.data(), vel.data());
+}
+
+extern "C" int bs_get_state_f64(bs_engine* e, double* pos3, double* vel3)
+{
+    if (!e) return BS_E_ARG;
+    const uint32_t n = e->prm.n;
+    if (n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_get_state_f64: NULL array");
+    if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        std::vector<float> p(3 * (size_t)n), v(3 * (size_t)n);
+        int rc = bs_get_state_f32(e, p.data(), v.data());
+        if (rc) return rc;
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) { pos3[k] = p[k]; vel3[k] = v[k]; }
+        return BS_OK;
+    }
+    HIP_TRY(e, launch_unsort_f64(n, e->da, e->tmp_pos, e->tmp_vel, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(pos3, e->tmp_pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(vel3, e->tmp_vel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BS_OK;
+}
+
+extern "C" int bs_get_state_f32(bs_engine* e, float* pos3, float* vel3)
+{
+    if (!e) return BS_E_ARG;
+    const uint32_t n = e->prm.n;
+    if (n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_get_state_f32: NULL array");
+    if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        std::vector<float> buf(6 * (size_t)n);
+        float* src[6] = {e->fa.x, e->fa.y, e->fa.z, e->fa.vx, e->fa.vy, e->fa.vz};
+        for (int k = 0; k < 6; ++k)
+            HIP_TRY(e, hipMemcpyAsync(buf.data() + (size_t)k * n, src[k], sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                pos3[3 * (size_t)i + k] = buf[(size_t)k * n + i];
+                vel3[3 * (size_t)i + k] = buf[(size_t)(3 + k) * n + i];
+            }
+        return BS_OK;
+    }
+    std::vector<double> p(3 * (size_t)n), v(3 * (size_t)n);
+    int rc = bs_get_state_f64(e, p.data(), v.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < 3 * (size_t)n; ++k) { pos3[k] = (float)p[k]; vel3[k] = (float)v[k]; }
+    return BS_OK;
+}
+
+extern "C" int bs_get_forces_f64(bs_engine* e, double* f9)
+{
+    if (!e) return BS_E_ARG;
+    const uint32_t n = e->prm.n;
+    if (!e->forces) return fail(e, BS_E_STATE, "engine created without BS_FLAG_FORCES");
+    if (n && !f9) return fail(e, BS_E_ARG, "bs_get_forces_f64: NULL array");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        std::vector<float> tmp(9 * (size_t)n);
+        HIP_TRY(e, hipMemcpyAsync(tmp.data(), e->forces, sizeof(float) * 9 * n, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        for (size_t k = 0; k < tmp.size(); ++k) f9[k] = tmp[k];
+        return BS_OK;
+    }
+    HIP_TRY(e, hipMemcpyAsync(f9, e->forces, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BS_OK;
+}
+
+extern "C" int bs_get_forces_f32(bs_engine* e, float* f9)
+{
+    if (!e) return BS_E_ARG;
+    const uint32_t n = e->prm.n;
+    if (!e->forces) return fail(e, BS_E_STATE, "engine created without BS_FLAG_FORCES");
+    if (n && !f9) return fail(e, BS_E_ARG, "bs_get_forces_f32: NULL array");
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+        HIP_TRY(e, hipMemcpyAsync(f9, e->forces, sizeof(float) * 9 * n, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        return BS_OK;
+    }
+    std::vector<double> tmp(9 * (size_t)n);
+    int rc = bs_get_forces_f64(e, tmp.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < tmp.size(); ++k) f9[k] = (float)tmp[k];
+    return BS_OK;
+}
+
+extern "C" int bs_get_digest(bs_engine* e, uint64_t* d4)
+{
+    if (!e) return BS_E_ARG;
+    if (!e->digest) return fail(e, BS_E_STATE, "engine created without BS_FLAG_DIGEST");
+    if (e->prm.n && !d4) return fail(e, BS_E_ARG, "bs_get_digest: NULL array");
+    HIP_TRY(e, hipMemcpyAsync(d4, e->digest, 32 * (size_t)e->prm.n, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BS_OK;
+}
+
+// ----------------------------------------------------------------------------
+// the step
+// ----------------------------------------------------------------------------
+extern "C" int bs_step(bs_engine* e, uint32_t nsteps)
+{
+    if (!e) return BS_E_ARG;
+    if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        switch (e->mode) {
+        case BS_MODE_ALLPAIRS_F32REF: {
+            Timed t(e, BS_K_PAIR);
+            HIP_TRY(e, launch_allpairs_f32ref(e->P, e->fa, e->fb, (float*)e->forces, e->digest, e->stream));
+            t.stop();
+            std::swap(e->fa, e->fb);
+            break;
+        }
+        case BS_MODE_ALLPAIRS_F64: {
+            Timed t(e, BS_K_PAIR);
+            HIP_TRY(e, launch_allpairs_f64(e->P, e->da, e->db, (double*)e->forces, e->digest, e->stream));
+            t.stop();
+            std::swap(e->da, e->db);   // both sides share one identity id array
+            break;
+        }
+        case BS_MODE_GRID_F64:
+        case BS_MODE_GRID_MIXED: {
+            Timed t(e, BS_K_PAIR);
+            if (e->mode == BS_MODE_GRID_F64)
+                HIP_TRY(e, launch_nbr_f64(e->P, e->da, e->db, e->key_sorted, e->start, e->table, e->nchunks,
+                                          e->max_chunks, e->key, e->count, (double*)e->forces, e->digest, e->stream));
+            else
+                HIP_TRY(e, launch_nbr_mixed(e->P, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
+                                            e->nchunks, e->max_chunks, e->key, e->count, (double*)e->forces,
+                                            e->digest, e->recheck, e->stream));
+            t.stop();
+            int rc = grid_sort_into_a(e);
+            if (rc) return rc;
+            break;
+        }
+        default:
+            return fail(e, BS_E_STATE, "bad mode");
+        }
+        e->steps++;
+    }
+    return BS_OK;
+}
+
+extern "C" int bs_sync(bs_engine* e)
+{
+    if (!e) return BS_E_ARG;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return resolve_timing(e);
+}
+
+extern "C" int bs_get_info(const bs_engine* e, bs_info* out)
+{
+    if (!e || !out) return BS_E_ARG;
+    out->mode = e->mode;
+    out->cells_per_axis = e->G;
+    out->cell_edge = e->edge;
+    out->device_bytes = e->device_bytes;
+    out->steps = e->steps;
+    return BS_OK;
+}
+
+extern "C" int bs_get_timing(bs_engine* e, int32_t kernel, double* ms, uint64_t* launches)
+{
+    if (!e || kernel < 0 || kernel >= BS_K_COUNT) return BS_E_ARG;
+    int rc = resolve_timing(e);
+    if (rc) return rc;
+    if (ms) *ms = e->ms[kernel];
+    if (launches) *launches = e->launches[kernel];
+    return BS_OK;
+}
+
+extern "C" int bs_reset_timing(bs_engine* e)
+{
+    if (!e) return BS_E_ARG;
+    int rc = resolve_timing(e);
+    if (rc) return rc;
+    for (int k = 0; k < BS_K_COUNT; ++k) { e->ms[k] = 0.0; e->launches[k] = 0; }
+    return BS_OK;
+}
+
+extern "C" int bs_debug_thresholds(const bs_params* p, double* out6)
+{
+    if (!p || !out6) return fail(nullptr, BS_E_ARG, "bs_debug_thresholds: NULL");
+    DevParams P{};
+    std::string why;
+    int rc = derive_thresholds(p, &P, &why);
+    if (rc) return fail(nullptr, rc, why);
+    const bool f32 = p->mode == BS_MODE_ALLPAIRS_F32REF;
+    for (int k = 0; k < 3; ++k) {
+        out6[k] = f32 ? (double)P.s_thr_f[k] : P.s_thr[k];
+        out6[3 + k] = P.cone[k] ? (f32 ? (double)P.c_thr_f[k] : P.c_thr[k]) : -2.0;
+    }
+    return BS_OK;
+}
+
+// Recheck counter of the mixed kernel (pairs decided in FP64), for reports.
+extern "C" int bs_debug_rechecks(bs_engine* e, uint64_t* out)
+{
+    if (!e || !out) return BS_E_ARG;
+    *out = 0;
+    if (!e->recheck) return BS_OK;
+    uint32_t v = 0;
+    HIP_TRY(e, hipMemcpyAsync(&v, e->recheck, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    *out = v;
+    return BS_OK;
+}

@@ -1,0 +1,91 @@
+// boids_internal.h — types shared by the engine host code and the gfx950
+// kernels.  Not part of the public ABI (see include/boids_engine.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace boids {
+
+// Rule slots, in BoidSimulation::setup order (src/boid_simulation.h:19-33).
+enum { SEP = 0, ALI = 1, COH = 2 };
+
+constexpr int kBlock = 256;        // threads per block of every kernel
+constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
+constexpr int kCapF64 = 512;       // LDS candidate capacity per pass (FP64 kernel)
+constexpr int kCapMixed = 1024;    // LDS candidate capacity per pass (mixed kernel)
+
+// Everything a kernel needs about the rules, by value in the kernarg segment.
+struct DevParams {
+    uint32_t n;
+    int32_t G;                 // cells per axis (grid modes)
+    double F, half;            // field edge, F/2
+    double inv_edge;           // G / F
+    // double-precision exact thresholds (FP64 modes and mixed rechecks)
+    double s_thr[3];           // d < r  <=>  s < s_thr   (s = squared distance, no FMA)
+    double c_thr[3];           // acos(c) <= pi*a  <=>  c >= c_thr && c <= 1
+    double s_max;              // max of s_thr
+    double w[3];
+    double vmax, vmin;
+    // float exact thresholds (reference-faithful float mode)
+    float s_thr_f[3], c_thr_f[3], s_max_f;
+    float w_f[3];
+    float vmax_f, vmin_f;
+    int32_t cone[3];           // 1: the rule has a view cone (a < 1); 0: distance only
+    int32_t never[3];          // 1: the cone accepts nothing (a < 0 or NaN)
+    // mixed precision: positions as 32-bit fixed-point lattice coordinates
+    // (unit = F / 2^32), FP32 pair tests with guard bands
+    double lat_scale;          // 2^32 / F
+    double unit;               // F / 2^32
+    float s_lo[3], s_hi[3];    // squared lattice distance: < s_lo surely in, >= s_hi surely out
+    float s_hi_max;
+    float c_lo[3], c_hi[3];    // c32 <= c_lo - k/d surely out, c32 >= c_hi + k/d surely in
+    float c_top;               // c32 >= c_top: too close to 1, recheck
+    float band_k;              // k: cosine band growth with 1/d (lattice units)
+};
+
+struct StateF64 {              // SoA, sorted (grid) or id order (all-pairs)
+    double *x, *y, *z, *vx, *vy, *vz;
+    uint32_t* id;
+};
+
+struct StateF32 {
+    float *x, *y, *z, *vx, *vy, *vz;
+};
+
+struct Lattice {               // mixed mode companion of the sorted FP64 state
+    uint4* p;                  // x, y, z lattice coordinates, w = sorted slot
+    float *vx, *vy, *vz;       // FP32 copies of the velocity
+};
+
+// Launchers (boids_kernels.hip).  All launch on `stream`.
+hipError_t launch_allpairs_f32ref(const DevParams& P, StateF32 in, StateF32 out, float* forces,
+                                  uint64_t* digest, hipStream_t stream);
+hipError_t launch_allpairs_f64(const DevParams& P, StateF64 in, StateF64 out, double* forces,
+                               uint64_t* digest, hipStream_t stream);
+hipError_t launch_hash_count(const DevParams& P, StateF64 st, uint32_t* key, uint32_t* count,
+                             hipStream_t stream);
+hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
+                       hipStream_t stream);
+hipError_t launch_rows(const DevParams& P, const uint32_t* start, uint4* table, uint32_t* nchunks,
+                       hipStream_t stream);
+hipError_t launch_scatter(const DevParams& P, const uint32_t* key, const uint32_t* start,
+                          uint32_t* count, const uint32_t* id, uint32_t* perm, uint32_t* permid,
+                          hipStream_t stream);
+hipError_t launch_gather(const DevParams& P, const uint32_t* perm, const uint32_t* permid,
+                         const uint32_t* key, const uint32_t* start, uint32_t* count,
+                         StateF64 src, StateF64 dst, uint32_t* key_sorted, Lattice* lat,
+                         hipStream_t stream);
+hipError_t launch_nbr_f64(const DevParams& P, StateF64 in, StateF64 out, const uint32_t* key_sorted,
+                          const uint32_t* start, const uint4* table, const uint32_t* nchunks,
+                          uint32_t max_chunks, uint32_t* key_out, uint32_t* count, double* forces,
+                          uint64_t* digest, hipStream_t stream);
+hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Lattice lat,
+                            const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
+                            const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out,
+                            uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
+                            hipStream_t stream);
+hipError_t launch_unsort_f64(uint32_t n, StateF64 src, double* pos3, double* vel3,
+                             hipStream_t stream);
+
+}  // namespace boids

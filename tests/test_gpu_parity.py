@@ -1,0 +1,264 @@
+"""Parity of the HIP engine (through its C ABI) with the reference and the oracle.
+
+Bars:
+  * float mode (BS_MODE_ALLPAIRS_F32REF): bit-identical to the reference's
+    own outputs (golden fixtures written by the compiled reference).
+  * FP64 modes: neighbour sets bit-exact with the FP64 oracle every step
+    (digests, GPU state fed to the oracle), positions/velocities within
+    1e-12 after one step and 1e-9 after ten (errors scaled by F and vmax).
+  * mixed mode: neighbour sets bit-exact with the FP64 oracle every step;
+    per-step error against FP64 within 1e-6 (scaled by F and vmax).
+"""
+import hashlib
+import os
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+import boidssimulatormpi_amd as B
+from boidssimulatormpi_amd import engine as E
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG0 = dict(sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008), vmax=0.005, vmin=0.001)
+SWEEP = dict(sep=(0.01, 0.5, 0.002), ali=(0.05, 1 / 3, 0.06), coh=(0.05, 0.5, 0.008), vmax=0.005, vmin=0.001)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rules_of(arr):
+    F, sr, sa, sw, ar, aa, aw, cr, ca, cw, vmax, vmin = [float(x) for x in arr]
+    return F, dict(sep=(sr, sa, sw), ali=(ar, aa, aw), coh=(cr, ca, cw), vmax=vmax, vmin=vmin)
+
+
+def make(n, F, rules, mode, flags=0, cells=0):
+    say(f"engine n={n} F={F} mode={E.MODE_NAMES[mode]} flags={flags}")
+    return B.Engine(B.make_params(n, F, **rules, mode=mode, flags=flags, cells_per_axis=cells))
+
+
+def wvec(rules):
+    return np.float32([rules["coh"][2]] * 3 + [rules["sep"][2]] * 3 + [rules["ali"][2]] * 3)
+
+
+def scaled_err(pa, va, pb, vb, F, vmax):
+    dp = pa - pb
+    dp -= F * np.round(dp / F)          # a wrap decided differently is the same point
+    return float(np.abs(dp).max() / F), float(np.abs(va - vb).max() / vmax)
+
+
+def iso_state(n, F, seed, rules=CFG0):
+    return B.generate_init(E.INIT_ISOTROPIC, n, F, rules["vmax"], rules["vmin"], seed=seed)
+
+
+# ------------------------------------------------ float mode vs reference ---
+def test_f32ref_random_uniform(golden_dir):
+    g = np.load(os.path.join(golden_dir, "ref_allpairs_ru_n1024.npz"))
+    F, rules = rules_of(g["rules"])
+    e = make(1024, F, rules, E.MODE_ALLPAIRS_F32REF, E.FLAG_FORCES)
+    e.set_state(xyzv6=np.concatenate([g["pos0"], g["vel0"]], 1))
+    for t in range(1, 101):
+        e.step(1)
+        if t in (1, 100):
+            p, v = e.state_f32()
+            assert np.array_equal(p, g[f"pos{t}"]) and np.array_equal(v, g[f"vel{t}"]), t
+            assert np.array_equal(e.forces_f32() * wvec(rules), g[f"force{t}"]), t
+
+
+def test_f32ref_file_init(golden_dir):
+    g = np.load(os.path.join(golden_dir, "ref_allpairs_file_n1024.npz"))
+    F, rules = rules_of(g["rules"])
+    e = make(1024, F, rules, E.MODE_ALLPAIRS_F32REF, E.FLAG_FORCES)
+    e.set_state(xyzv6=g["init"])
+    for t in range(1, 101):
+        e.step(1)
+        if t in (1, 10, 100):
+            p, v = e.state_f32()
+            assert np.array_equal(p, g[f"pos{t}"]) and np.array_equal(v, g[f"vel{t}"]), t
+        if t in (1, 100):
+            assert np.array_equal(e.forces_f32() * wvec(rules), g[f"force{t}"]), t
+
+
+@pytest.mark.parametrize("name,steps", [("ref_grid_ru_seed1_n2048.npz", 50), ("ref_grid_distonly_n2048.npz", 20)])
+def test_f32ref_matches_reference_grid(golden_dir, name, steps):
+    g = np.load(os.path.join(golden_dir, name))
+    F, rules = rules_of(g["rules"])
+    e = make(2048, F, rules, E.MODE_ALLPAIRS_F32REF)
+    e.set_state(xyzv6=np.concatenate([g["pos0"], g["vel0"]], 1))
+    e.step(steps)
+    p, v = e.state_f32()
+    assert np.array_equal(p, g[f"pos{steps}"]) and np.array_equal(v, g[f"vel{steps}"])
+
+
+def test_f32ref_testini_ptcl(golden_dir):
+    g = np.load(os.path.join(golden_dir, "ref_allpairs_testini_n1000.npz"))
+    F, rules = rules_of(g["rules"])
+    e = make(1000, F, rules, E.MODE_ALLPAIRS_F32REF)
+    e.set_state(xyzv6=np.concatenate([g["pos0"], g["vel0"]], 1))
+    hdr = struct.pack("<4sBBIIIIIffffff", b"PTCL", 0, 2, 50, 1000, 30, 0, 30, 0.0, F, 0.0, F, 0.0, F)
+    hp, hv = hashlib.sha256(hdr), hashlib.sha256(hdr)
+    for _ in range(30):
+        p, v = e.state_f32()
+        hp.update(p.astype("<f4").tobytes()); hv.update(v.astype("<f4").tobytes())
+        e.step(1)
+    assert hp.digest() == g["ptcl_sha256"].tobytes() and hv.digest() == g["vptcl_sha256"].tobytes()
+
+
+# ------------------------------------------------------- FP64 all-pairs ---
+def test_f64_allpairs_bit_exact_with_oracle():
+    n, F = 1024, 1.0
+    st = iso_state(n, F, 0)
+    e = make(n, F, CFG0, E.MODE_ALLPAIRS_F64, E.FLAG_FORCES | E.FLAG_DIGEST)
+    e.set_state(xyzv6=st)
+    P = O.params(F, **CFG0)
+    p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    for t in range(10):
+        f, d = O.step(P, p, v, forces=True, digest=True)
+        e.step(1)
+        gp, gv = e.state_f64()
+        assert np.array_equal(e.digest(), d), t
+        assert np.array_equal(gp, p) and np.array_equal(gv, v), t   # same order of sums: same bits
+        assert np.array_equal(e.forces_f64(), f), t
+
+
+# -------------------------------------------------------------- grid ---
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+@pytest.mark.parametrize("n,F,rules", [(16384, 1.0, CFG0), (4096, 1.0, SWEEP), (20000, 1.6, CFG0)])
+def test_grid_neighbour_sets_each_step(mode, n, F, rules):
+    st = iso_state(n, F, 1, rules)
+    e = make(n, F, rules, mode, E.FLAG_DIGEST | E.FLAG_FORCES)
+    assert e.info().mode == mode and e.info().cells_per_axis >= 3
+    e.set_state(xyzv6=st)
+    P = O.params(F, **rules)
+    gp, gv = e.state_f64()
+    worst = (0.0, 0.0)
+    for t in range(6):
+        p, v = gp.copy(), gv.copy()
+        f, d = O.step(P, p, v, forces=True, digest=True, enumeration=O.ENUM_CELLS)
+        e.step(1)
+        gd = e.digest()
+        bad = np.nonzero((gd != d).any(1))[0]
+        assert bad.size == 0, f"step {t}: {bad.size} boids with different neighbour sets, e.g. {bad[:5]}"
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, rules["vmax"])
+        worst = (max(worst[0], ep), max(worst[1], ev))
+        tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+        assert ep <= tol and ev <= tol, (t, ep, ev)
+    say(f"per-step max scaled error pos={worst[0]:.3e} vel={worst[1]:.3e} rechecks={e.rechecks()}")
+
+
+def test_grid_f64_ten_steps_continuous():
+    n, F = 16384, 1.0
+    st = iso_state(n, F, 1)
+    e = make(n, F, CFG0, E.MODE_GRID_F64)
+    e.set_state(xyzv6=st)
+    P = O.params(F, **CFG0)
+    p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    for t in range(1, 11):
+        O.step(P, p, v, enumeration=O.ENUM_CELLS)
+        e.step(1)
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
+        tol = 1e-12 if t == 1 else 1e-9
+        assert ep <= tol and ev <= tol, (t, ep, ev)
+
+
+def test_grid_deterministic_and_resume_equals_continuous():
+    n, F = 8192, 1.0
+    st = iso_state(n, F, 5)
+    a = make(n, F, CFG0, E.MODE_GRID_F64)
+    a.set_state(xyzv6=st)
+    a.step(10)
+    pa, va = a.state_f64()
+    b = make(n, F, CFG0, E.MODE_GRID_F64)
+    b.set_state(xyzv6=st)
+    b.step(4)
+    p4, v4 = b.state_f64()
+    c = make(n, F, CFG0, E.MODE_GRID_F64)
+    c.set_state(pos=p4, vel=v4)          # sorted order depends only on the state
+    c.step(6)
+    pc, vc = c.state_f64()
+    assert np.array_equal(pa, pc) and np.array_equal(va, vc)
+
+
+def test_cfg2_full_size_one_step():
+    # N = 2^20, F = 4 (density 16384): neighbour sets of one step, both modes
+    n, F = 1 << 20, 4.0
+    st = iso_state(n, F, 2)
+    P = O.params(F, **CFG0)
+    p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    say("oracle step at N=2^20 ...")
+    _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+    for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
+        e = make(n, F, CFG0, mode, E.FLAG_DIGEST)
+        e.set_state(xyzv6=st)
+        e.step(1)
+        gd = e.digest()
+        assert np.array_equal(gd, d), E.MODE_NAMES[mode]
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
+        assert ep <= (1e-12 if mode == E.MODE_GRID_F64 else 1e-6), ep
+        assert ev <= (1e-12 if mode == E.MODE_GRID_F64 else 1e-6), ev
+        # size-independent properties after more steps
+        e.step(4)
+        gp, gv = e.state_f64()
+        s = np.linalg.norm(gv, axis=1)
+        assert np.isfinite(gp).all() and (gp >= 0).all() and (gp <= F).all()
+        assert (s >= CFG0["vmin"] * (1 - 1e-12)).all() and (s <= CFG0["vmax"] * (1 + 1e-12)).all()
+        say(f"{E.MODE_NAMES[mode]} ok, rechecks={e.rechecks()}")
+
+
+# ---------------------------------------------------------- edge cases ---
+@pytest.mark.parametrize("mode", [E.MODE_ALLPAIRS_F32REF, E.MODE_ALLPAIRS_F64, E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+def test_tiny_and_degenerate_inputs(mode):
+    F = 1.0
+    # n = 0
+    e = make(0, F, CFG0, mode)
+    e.set_state(xyzv6=np.zeros((0, 6), np.float32))
+    e.step(3)
+    # n = 1 crossing the wall; n = 4 with a coincident pair and a zero velocity
+    st = np.array([[0.001, 0.5, 0.5, -0.004, 0.0, 0.0]], np.float32)
+    e = make(1, F, CFG0, mode)
+    e.set_state(xyzv6=st)
+    e.step(1)
+    p, v = e.state_f64()
+    assert abs(p[0, 0] - (1.0 + (0.001 - 0.004))) < 1e-7 and abs(v[0, 0] + 0.004) < 1e-9
+    st = np.array([[0.5, 0.5, 0.5, 0.003, 0, 0], [0.5, 0.5, 0.5, 0.003, 0, 0],
+                   [0.51, 0.5, 0.5, 0.0, 0.0, 0.0], [0.995, 0.5, 0.5, 0.002, 0.001, 0]], np.float32)
+    e = make(4, F, CFG0, mode, E.FLAG_DIGEST)
+    e.set_state(xyzv6=st)
+    e.step(1)
+    P = O.params(F, **CFG0)
+    if mode == E.MODE_ALLPAIRS_F32REF:
+        p, v = st[:, :3].copy(), st[:, 3:].copy()
+        _, d = O.step(P, p, v, digest=True)
+        gp, gv = e.state_f32()
+        assert np.array_equal(gp, p) and np.array_equal(gv, v)
+    else:
+        p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+        _, d = O.step(P, p, v, digest=True)
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
+        assert ep < 1e-9 and ev < 1e-9
+    assert np.array_equal(e.digest(), d)
+
+
+def test_small_field_falls_back_to_allpairs():
+    # F / r_max < 3 cells per axis: the grid modes run all-pairs
+    n, F = 500, 0.2
+    st = iso_state(n, F, 6)
+    e = make(n, F, CFG0, E.MODE_GRID_F64)
+    assert e.info().mode == E.MODE_ALLPAIRS_F64
+    e.set_state(xyzv6=st)
+    e.step(3)
+    P = O.params(F, **CFG0)
+    p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    for _ in range(3):
+        O.step(P, p, v)
+    gp, gv = e.state_f64()
+    assert np.array_equal(gp, p) and np.array_equal(gv, v)
