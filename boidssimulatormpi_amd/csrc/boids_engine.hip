@@ -829,3 +829,34 @@ extern "C" int bs_debug_rechecks(bs_engine* e, uint64_t* out)
     *out = v;
     return BS_OK;
 }
+
+// Known-size streaming copy on the default device, for calibrating HBM byte
+// counters: `reps` launches each moving `bytes` in and `bytes` out.
+extern "C" int bs_debug_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
+{
+    if (bytes < 16 || reps <= 0) return fail(nullptr, BS_E_ARG, "bs_debug_copy: bad arguments");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = BS_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = fail(nullptr, BS_E_NOMEM, "hipMalloc failed");
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = fail(nullptr, BS_E_HIP, "stream/event creation failed");
+    if (!rc && hipMemsetAsync(a, 1, bytes, st) != hipSuccess) rc = fail(nullptr, BS_E_HIP, "memset failed");
+    if (!rc) {
+        (void)hipEventRecord(e0, st);
+        for (int k = 0; k < reps && !rc; ++k)
+            if (launch_copy(a, b, bytes, st) != hipSuccess) rc = fail(nullptr, BS_E_HIP, "copy launch failed");
+        (void)hipEventRecord(e1, st);
+        if (hipStreamSynchronize(st) != hipSuccess) rc = fail(nullptr, BS_E_HIP, "copy failed");
+        float ms = 0.f;
+        if (!rc && ms_per_copy && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) *ms_per_copy = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}

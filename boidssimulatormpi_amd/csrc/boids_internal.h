@@ -85,6 +85,7 @@ hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Latti
                             const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
                             hipStream_t stream);
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
 hipError_t launch_unsort_f64(uint32_t n, StateF64 src, double* pos3, double* vel3,
                              hipStream_t stream);
 

@@ -877,6 +877,25 @@ hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Latti
     return hipGetLastError();
 }
 
+// Plain streaming copy (16 B per lane per iteration): the reference point
+// for HBM byte counters (a copy of B bytes moves exactly B in and B out).
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                 uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream)
+{
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return hipSuccess;
+    const uint64_t want = (n16 + kBlock - 1) / kBlock;
+    const uint32_t grid = (uint32_t)(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, stream, (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
 hipError_t launch_unsort_f64(uint32_t n, StateF64 src, double* pos3, double* vel3,
                              hipStream_t stream)
 {

@@ -70,7 +70,7 @@ EXPORTS = ["bs_create", "bs_destroy", "bs_last_error", "bs_generate_init", "bs_r
            "bs_set_state_f32", "bs_set_state_f64", "bs_step", "bs_sync", "bs_get_state_f32",
            "bs_get_state_f64", "bs_get_forces_f32", "bs_get_forces_f64", "bs_get_digest",
            "bs_get_info", "bs_get_timing", "bs_reset_timing", "bs_debug_thresholds",
-           "bs_debug_rechecks", "bs_version"]
+           "bs_debug_rechecks", "bs_debug_copy", "bs_version"]
 
 
 def lib():
@@ -104,6 +104,7 @@ def lib():
         "bs_reset_timing": (c_int, [vp]),
         "bs_debug_thresholds": (c_int, [P(Params), f64p]),
         "bs_debug_rechecks": (c_int, [vp, u64p]),
+        "bs_debug_copy": (c_int, [ctypes.c_uint64, ctypes.c_int32, f64p]),
         "bs_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -140,6 +141,15 @@ def debug_thresholds(p: Params) -> np.ndarray:
     if rc != 0:
         raise EngineError(lib().bs_last_error(None).decode())
     return out
+
+
+def debug_copy(nbytes: int, reps: int = 1) -> float:
+    """Runs `reps` plain device copies of `nbytes`; returns ms per copy."""
+    ms = ctypes.c_double()
+    rc = lib().bs_debug_copy(nbytes, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise EngineError(lib().bs_last_error(None).decode())
+    return ms.value
 
 
 class Engine:

@@ -165,6 +165,11 @@ int bs_debug_thresholds(const bs_params* p, double* out6);
  * creation (guard-band hits).  0 for other modes. */
 int bs_debug_rechecks(bs_engine* e, uint64_t* out);
 
+/* Measurement helper: `reps` launches of a plain streaming copy of `bytes`
+ * bytes on the current device (reference point for HBM byte counters).
+ * *ms_per_copy receives the average HIP-event time per copy. */
+int bs_debug_copy(uint64_t bytes, int32_t reps, double* ms_per_copy);
+
 /* is_openmp_enabled()/get_max_threads() have no device counterpart; the
  * shim answers them.  Library version string. */
 const char* bs_version(void);
