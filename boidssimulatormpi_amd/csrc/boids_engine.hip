@@ -268,6 +268,21 @@ static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
     }
     P->c_top = round_down_f(1.0 - beta_c);
     P->band_k = 8.0f;   // >= 2*sqrt(3) lattice units of displacement error, x2 margin
+    // Coarse filter on the top 16 lattice bits (unit F/2^16): a coarse
+    // difference is off by at most one unit per axis, so every pair with
+    // d < max(r) + beta_d has coarse squared distance < (r16 + 2)^2.
+    {
+        double rhi = 0.0;
+        bool unbounded = false;
+        for (int k = 0; k < 3; ++k) {
+            if (isnan(r[k])) continue;
+            if (isinf(r[k])) { unbounded = true; continue; }
+            rhi = std::max(rhi, r[k] + 1e-5 + 2e-6 * fabs(r[k]));
+        }
+        const double r16 = rhi / (p->field / 65536.0);
+        const double thr = ceil((r16 + 2.0) * (r16 + 2.0));
+        P->coarse_thr = (unbounded || !(thr < 2147483647.0)) ? 2147483647 : (int32_t)thr;
+    }
     P->vmax = p->vmax;
     P->vmin = p->vmin;
     P->vmax_f = (float)p->vmax;

@@ -12,8 +12,15 @@ enum { SEP = 0, ALI = 1, COH = 2 };
 
 constexpr int kBlock = 256;        // threads per block of every kernel
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
-constexpr int kCapF64 = 512;       // LDS candidate capacity per pass (FP64 kernel)
-constexpr int kCapMixed = 1024;    // LDS candidate capacity per pass (mixed kernel)
+constexpr int kCapF64 = 384;       // LDS candidate capacity per pass (FP64 kernel)
+constexpr int kCapMixed = 512;     // LDS candidate capacity per pass (mixed kernel)
+#ifndef BS_QUEUE_DEPTH
+#define BS_QUEUE_DEPTH 16
+#endif
+#ifndef BS_NBR_QUEUE
+#define BS_NBR_QUEUE 1             // 1: filter -> per-lane queue -> dense pass; 0: evaluate inline
+#endif
+constexpr int kQueue = BS_QUEUE_DEPTH;   // per-lane queue of in-range candidates (LDS indices)
 
 // Everything a kernel needs about the rules, by value in the kernarg segment.
 struct DevParams {
@@ -42,6 +49,7 @@ struct DevParams {
     float c_lo[3], c_hi[3];    // c32 <= c_lo - k/d surely out, c32 >= c_hi + k/d surely in
     float c_top;               // c32 >= c_top: too close to 1, recheck
     float band_k;              // k: cosine band growth with 1/d (lattice units)
+    int32_t coarse_thr;        // squared distance bound on the top 16 lattice bits
 };
 
 struct StateF64 {              // SoA, sorted (grid) or id order (all-pairs)

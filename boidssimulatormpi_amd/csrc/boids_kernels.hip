@@ -17,6 +17,8 @@
 
 namespace boids {
 
+typedef short v2s __attribute__((ext_vector_type(2)));   // packed int16 pair
+
 __device__ __forceinline__ uint64_t mix64(uint64_t j)
 {
     uint64_t z = j + 0x9E3779B97F4A7C15ull;
@@ -472,6 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
 {
     __shared__ double sx[kCapF64], sy[kCapF64], sz[kCapF64], svx[kCapF64], svy[kCapF64], svz[kCapF64];
     __shared__ uint32_t sid[DIGEST ? kCapF64 : 1];
+    __shared__ uint16_t sq[kQueue * kBlock];   // per-lane queues, lane-interleaved
     __shared__ Segment S;
 
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
@@ -499,6 +502,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
     }
     const double vn = norm3(vx, vy, vz);
     const double F = P.F;
+    // phase-1 bound: a fused s2 differs from the unfused one by < 1e-15 relative
+    const double s_filter = P.s_max * (1.0 + 1e-12);
     double scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
     int nc = 0, ns = 0, na = 0;
     uint64_t hc = 0, hs = 0, ha = 0;
@@ -533,37 +538,63 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
             }
             __syncthreads();
             if (!active) continue;
+            // Full rule evaluation of one candidate inside the largest radius.
+            auto pair = [&](uint32_t kk) {
+                    const double qx = sx[kk], qy = sy[kk], qz = sz[kk];
+                    const double ex = px - qx, ey = py - qy, ez = pz - qz;
+                    const double s2 = ex * ex + ey * ey + ez * ez;
+                    bool in_c = s2 < P.s_thr[COH], in_s = s2 < P.s_thr[SEP], in_a = s2 < P.s_thr[ALI];
+                    if ((in_c && k_c) || (in_s && k_s) || (in_a && k_a)) {
+                        const double gx = qx - px, gy = qy - py, gz = qz - pz;
+                        const double dot = vx * gx + vy * gy + vz * gz;
+                        const double c = dot / (vn * sqrt(s2));
+                        if (k_c) in_c = in_c && c >= P.c_thr[COH] && c <= 1.0;
+                        if (k_s) in_s = in_s && c >= P.c_thr[SEP] && c <= 1.0;
+                        if (k_a) in_a = in_a && c >= P.c_thr[ALI] && c <= 1.0;
+                    }
+                    if (in_c) {
+                        nc++; scx += qx; scy += qy; scz += qz;
+                        if (DIGEST) hc += mix64(sid[kk]);
+                    }
+                    if (in_s) {
+                        const double t = sqrt(s2);
+                        ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t;
+                        if (DIGEST) hs += mix64(sid[kk]);
+                    }
+                    if (in_a) {
+                        na++; sax += svx[kk]; say += svy[kk]; saz += svz[kk];
+                        if (DIGEST) ha += mix64(sid[kk]);
+                    }
+            };
+            // Phase 1: squared distance only (any rounding may be used here:
+            // the filter keeps a superset, the pair body re-decides exactly).
             const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
+#if BS_NBR_QUEUE
+            // Candidates inside the largest radius go to this lane's queue,
+            // drained (phase 2, densely packed) when any lane fills up.
+            auto drain = [&](uint32_t qn) {
+                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kBlock + threadIdx.x]);
+            };
+            uint32_t qn = 0;
             for (uint32_t k = kb; k < ke; ++k) {
                 const uint32_t kk = k - base;
-                const double qx = sx[kk], qy = sy[kk], qz = sz[kk];
-                const double ex = px - qx, ey = py - qy, ez = pz - qz;
-                const double s2 = ex * ex + ey * ey + ez * ez;
-                if (!(s2 < P.s_max)) continue;
-                if (k == selfk) continue;
-                bool in_c = s2 < P.s_thr[COH], in_s = s2 < P.s_thr[SEP], in_a = s2 < P.s_thr[ALI];
-                if ((in_c && k_c) || (in_s && k_s) || (in_a && k_a)) {
-                    const double gx = qx - px, gy = qy - py, gz = qz - pz;
-                    const double dot = vx * gx + vy * gy + vz * gz;
-                    const double c = dot / (vn * sqrt(s2));
-                    if (k_c) in_c = in_c && c >= P.c_thr[COH] && c <= 1.0;
-                    if (k_s) in_s = in_s && c >= P.c_thr[SEP] && c <= 1.0;
-                    if (k_a) in_a = in_a && c >= P.c_thr[ALI] && c <= 1.0;
+                const double ex = px - sx[kk], ey = py - sy[kk], ez = pz - sz[kk];
+                const double s2 = fma(ex, ex, fma(ey, ey, ez * ez));
+                if (s2 < s_filter && k != selfk) {
+                    sq[qn * kBlock + threadIdx.x] = (uint16_t)kk;
+                    ++qn;
                 }
-                if (in_c) {
-                    nc++; scx += qx; scy += qy; scz += qz;
-                    if (DIGEST) hc += mix64(sid[kk]);
-                }
-                if (in_s) {
-                    const double t = sqrt(s2);
-                    ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t;
-                    if (DIGEST) hs += mix64(sid[kk]);
-                }
-                if (in_a) {
-                    na++; sax += svx[kk]; say += svy[kk]; saz += svz[kk];
-                    if (DIGEST) ha += mix64(sid[kk]);
-                }
+                if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
             }
+            drain(qn);
+#else
+            for (uint32_t k = kb; k < ke; ++k) {
+                const uint32_t kk = k - base;
+                const double ex = px - sx[kk], ey = py - sy[kk], ez = pz - sz[kk];
+                const double s2 = fma(ex, ex, fma(ey, ey, ez * ez));
+                if (s2 < s_filter && k != selfk) pair(kk);
+            }
+#endif
         }
     }
     if (!active) return;
@@ -642,9 +673,11 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
                                                       double* forces, uint64_t* digest,
                                                       uint32_t* recheck)
 {
-    __shared__ uint4 sl[kCapMixed];
+    __shared__ uint4 sl[kCapMixed];          // lattice x, y, z, sorted slot
+    __shared__ uint2 sc[kCapMixed];          // top 16 bits of x|y, z: the coarse lattice
     __shared__ float svx[kCapMixed], svy[kCapMixed], svz[kCapMixed];
     __shared__ uint32_t sid[DIGEST ? kCapMixed : 1];
+    __shared__ uint16_t sq[kQueue * kBlock];
     __shared__ Segment S;
 
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
@@ -660,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
     const int nseg = cxb - cxa + 3;
 
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t myid = 0;
+    uint32_t myid = 0, mystart = 0;
     uint4 me = make_uint4(0, 0, 0, 0xFFFFFFFFu);
     float fvx = 0.f, fvy = 0.f, fvz = 0.f;
     int L = 1;
@@ -670,8 +703,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
         myid = in.id[s];
         me = lat.p[s];
         fvx = lat.vx[s]; fvy = lat.vy[s]; fvz = lat.vz[s];
-        L = (int)(key_sorted[s] % (uint32_t)G) - cxa + 1;
+        const uint32_t mykey = key_sorted[s];
+        mystart = start[mykey];
+        L = (int)(mykey % (uint32_t)G) - cxa + 1;
     }
+    const v2s my_xy = __builtin_bit_cast(v2s, (me.x >> 16) | (me.y & 0xFFFF0000u));
+    const v2s my_z = __builtin_bit_cast(v2s, me.z >> 16);
     const double vn = norm3(vx, vy, vz);
     const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
     // a vanishing FP32 velocity makes the FP32 cosine meaningless: recheck
@@ -694,6 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
         build_segment(S, start, G, cxa, nseg, nrow);
         const uint32_t total = S.segc[nseg];
         const uint32_t wlo = S.segc[L - 1], whi = S.segc[L + 2];
+        const uint32_t selfk = (rr == 4) ? S.segc[L] + (s - mystart) : 0xFFFFFFFFu;
         for (uint32_t base = 0; base < total; base += kCapMixed) {
             const uint32_t here = min((uint32_t)kCapMixed, total - base);
             __syncthreads();
@@ -701,22 +739,22 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
                 const uint32_t g = base + idx;
                 const int k = seg_cell(S, nseg, g);
                 const uint32_t slot = S.segg[k] + (g - S.segc[k]);
-                sl[idx] = lat.p[slot];
+                const uint4 p4 = lat.p[slot];
+                sl[idx] = p4;
+                sc[idx] = make_uint2((p4.x >> 16) | (p4.y & 0xFFFF0000u), p4.z >> 16);
                 svx[idx] = lat.vx[slot]; svy[idx] = lat.vy[slot]; svz[idx] = lat.vz[slot];
                 if (DIGEST) sid[idx] = in.id[slot];
             }
             __syncthreads();
             if (!active) continue;
-            const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
-            for (uint32_t k = kb; k < ke; ++k) {
-                const uint32_t kk = k - base;
+            // FP32 rule evaluation of one coarse-filtered candidate.
+            auto pair = [&](uint32_t kk) {
                 const uint4 q = sl[kk];
                 const float gx = (float)(int32_t)(q.x - me.x);
                 const float gy = (float)(int32_t)(q.y - me.y);
                 const float gz = (float)(int32_t)(q.z - me.z);
                 const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
-                if (!(s2 < P.s_hi_max)) continue;
-                if (q.w == s) continue;
+                if (!(s2 < P.s_hi_max)) return;
                 const float inv_d = __builtin_amdgcn_rsqf(s2);
                 const float c = fmaf(fvz, gz, fmaf(fvy, gy, fvx * gx)) * inv_d * inv_vn;
                 const float band = P.band_k * inv_d;
@@ -746,7 +784,38 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
                     kahan_add(sax, cax, svx[kk]); kahan_add(say, cay, svy[kk]); kahan_add(saz, caz, svz[kk]);
                     if (DIGEST) ha += mix64(sid[kk]);
                 }
+            };
+            // Phase 1: coarse squared distance on the top 16 lattice bits
+            // (packed int16 subtract + dot2, saturating); it under-estimates
+            // no distance by more than sqrt(3) coarse units, and coarse_thr
+            // carries that margin, so no candidate of any rule is dropped.
+            const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
+            auto coarse = [&](uint32_t kk) {
+                const uint2 cq = sc[kk];
+                const v2s dxy = __builtin_bit_cast(v2s, cq.x) - my_xy;
+                const v2s dzz = __builtin_bit_cast(v2s, cq.y) - my_z;
+                return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dzz, dzz, 0, true), true);
+            };
+#if BS_NBR_QUEUE
+            auto drain = [&](uint32_t qn) {
+                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kBlock + threadIdx.x]);
+            };
+            uint32_t qn = 0;
+            for (uint32_t k = kb; k < ke; ++k) {
+                const uint32_t kk = k - base;
+                if (coarse(kk) <= P.coarse_thr && k != selfk) {
+                    sq[qn * kBlock + threadIdx.x] = (uint16_t)kk;
+                    ++qn;
+                }
+                if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
             }
+            drain(qn);
+#else
+            for (uint32_t k = kb; k < ke; ++k) {
+                const uint32_t kk = k - base;
+                if (coarse(kk) <= P.coarse_thr && k != selfk) pair(kk);
+            }
+#endif
         }
     }
     if (nre) atomicAdd(recheck, nre);

@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libboids_engine.so")
+# BOIDS_ENGINE_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("BOIDS_ENGINE_LIB") or os.path.join(PKG_DIR, "libboids_engine.so")
 
 MODE_ALLPAIRS_F32REF = 0
 MODE_ALLPAIRS_F64 = 1
