@@ -515,7 +515,8 @@ extern "C" int bs_create(const bs_params* p, bs_engine** out)
     if (e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED) {
         const uint64_t G = (uint64_t)e->G;
         e->ncell1 = (uint32_t)(G * G * G + 1);
-        e->max_chunks = (n + kBlock - 1) / kBlock + (uint32_t)(G * G) + 1;
+        // sum over rows of ceil(n_row / kNbr) <= n / kNbr + rows
+        e->max_chunks = (n + kNbr - 1) / kNbr + (uint32_t)(G * G) + 1;
         const uint32_t ntiles = (e->ncell1 + 4095) / 4096;
         if ((rc = dev_alloc(e, &e->key, n)) || (rc = dev_alloc(e, &e->key_sorted, n)) ||
             (rc = dev_alloc(e, &e->count, e->ncell1)) || (rc = dev_alloc(e, &e->start, e->ncell1)) ||

@@ -10,7 +10,11 @@ namespace boids {
 // Rule slots, in BoidSimulation::setup order (src/boid_simulation.h:19-33).
 enum { SEP = 0, ALI = 1, COH = 2 };
 
-constexpr int kBlock = 256;        // threads per block of every kernel
+constexpr int kBlock = 256;        // threads per block of the streaming kernels
+#ifndef BS_NBR_BLOCK
+#define BS_NBR_BLOCK 256
+#endif
+constexpr int kNbr = BS_NBR_BLOCK;  // threads (= boids per chunk) of the neighbour kernels
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
 constexpr int kCapF64 = 384;       // LDS candidate capacity per pass (FP64 kernel)
 constexpr int kCapMixed = 512;     // LDS candidate capacity per pass (mixed kernel)

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rows(DevParams P, const uint32
         const uint32_t r = b + threadIdx.x;
         uint32_t lo = 0, nr = 0;
         if (r < rows) { lo = start[r * G]; nr = start[(r + 1) * G] - lo; }
-        const uint32_t kr = (nr + kBlock - 1) / kBlock;
+        const uint32_t kr = (nr + kNbr - 1) / kNbr;
         uint32_t total;
         const uint32_t off = block_exclusive_scan(kr, wsum, &total) + carry;
         if (kr) {
@@ -379,20 +379,36 @@ __global__ __launch_bounds__(kBlock) void k_unsort_f64(uint32_t n, StateF64 src,
 // cx-1..cx+1 of that segment.  Segments larger than the LDS capacity are
 // processed in passes.
 // ---------------------------------------------------------------------------
+// Segment tables live in dynamic LDS sized for G + 2 cells (segment_bytes).
 struct Segment {
-    uint32_t segc[kMaxCells + 3];   // LDS-index prefix over segment cells
-    uint32_t segg[kMaxCells + 2];   // global sorted start of each segment cell
-    int8_t segsh[kMaxCells + 2];    // x periodic shift of each segment cell: -1, 0, +1 (x F)
-    uint32_t wsum[17];
+    uint32_t* segc;    // [G+3] LDS-index prefix over segment cells
+    uint32_t* segg;    // [G+2] global sorted start of each segment cell
+    int8_t* segsh;     // [G+2] x periodic shift of each segment cell: -1, 0, +1 (x F)
+    uint32_t* wsum;    // [17] scan scratch
 };
+
+__host__ __device__ constexpr size_t segment_bytes(int G)
+{
+    return ((size_t)(G + 3) * 4 + (size_t)(G + 2) * 4 + (size_t)(G + 2) + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ Segment segment_view(unsigned char* dyn, int G, uint32_t* wsum)
+{
+    Segment S;
+    S.segc = (uint32_t*)dyn;
+    S.segg = S.segc + (G + 3);
+    S.segsh = (int8_t*)(S.segg + (G + 2));
+    S.wsum = wsum;
+    return S;
+}
 
 // Builds the segment of neighbour row `nrow` for cells cxa-1 .. cxa+nseg-2.
 __device__ __forceinline__ void build_segment(Segment& S, const uint32_t* start, int G, int cxa,
                                               int nseg, uint32_t nrow)
 {
     // counts, then a block-wide exclusive scan (each thread owns up to
-    // ceil(nseg/kBlock) consecutive cells)
-    const int per = (nseg + kBlock - 1) / kBlock;
+    // ceil(nseg/kNbr) consecutive cells)
+    const int per = (nseg + kNbr - 1) / kNbr;
     const int k0 = threadIdx.x * per;
     uint32_t local = 0;
     for (int k = k0; k < min(nseg, k0 + per); ++k) {
@@ -464,23 +480,25 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t grid)
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, StateF64 out,
-                                                    const uint32_t* __restrict__ key_sorted,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint4* __restrict__ table,
-                                                    const uint32_t* __restrict__ nchunks,
-                                                    uint32_t* key_out, uint32_t* count,
-                                                    double* forces, uint64_t* digest)
+__global__ __launch_bounds__(kNbr) void k_nbr_f64(DevParams P, StateF64 in, StateF64 out,
+                                                  const uint32_t* __restrict__ key_sorted,
+                                                  const uint32_t* __restrict__ start,
+                                                  const uint4* __restrict__ table,
+                                                  const uint32_t* __restrict__ nchunks,
+                                                  uint32_t* key_out, uint32_t* count,
+                                                  double* forces, uint64_t* digest)
 {
     __shared__ double sx[kCapF64], sy[kCapF64], sz[kCapF64], svx[kCapF64], svy[kCapF64], svz[kCapF64];
     __shared__ uint32_t sid[DIGEST ? kCapF64 : 1];
-    __shared__ uint16_t sq[kQueue * kBlock];   // per-lane queues, lane-interleaved
-    __shared__ Segment S;
+    __shared__ uint16_t sq[kQueue * kNbr];   // per-lane queues, lane-interleaved
+    __shared__ uint32_t wsum[17];
+    extern __shared__ __align__(16) unsigned char dyn[];
 
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= *nchunks) return;
     const uint4 ch = table[b];
     const int G = P.G;
+    Segment S = segment_view(dyn, G, wsum);
     const uint32_t row = ch.x, t0 = ch.y, t1 = ch.z;
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
     const uint32_t s = t0 + threadIdx.x;
@@ -524,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
         for (uint32_t base = 0; base < total; base += kCapF64) {
             const uint32_t here = min((uint32_t)kCapF64, total - base);
             __syncthreads();
-            for (uint32_t idx = threadIdx.x; idx < here; idx += kBlock) {
+            for (uint32_t idx = threadIdx.x; idx < here; idx += kNbr) {
                 const uint32_t g = base + idx;
                 const int k = seg_cell(S, nseg, g);
                 const uint32_t slot = S.segg[k] + (g - S.segc[k]);
@@ -573,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
             // Candidates inside the largest radius go to this lane's queue,
             // drained (phase 2, densely packed) when any lane fills up.
             auto drain = [&](uint32_t qn) {
-                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kBlock + threadIdx.x]);
+                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kNbr + threadIdx.x]);
             };
             uint32_t qn = 0;
             for (uint32_t k = kb; k < ke; ++k) {
@@ -581,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_f64(DevParams P, StateF64 in, St
                 const double ex = px - sx[kk], ey = py - sy[kk], ez = pz - sz[kk];
                 const double s2 = fma(ex, ex, fma(ey, ey, ez * ez));
                 if (s2 < s_filter && k != selfk) {
-                    sq[qn * kBlock + threadIdx.x] = (uint16_t)kk;
+                    sq[qn * kNbr + threadIdx.x] = (uint16_t)kk;
                     ++qn;
                 }
                 if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
@@ -663,27 +681,29 @@ __device__ __forceinline__ int rule_verdict(const DevParams& P, int r, float s, 
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, StateF64 out,
-                                                      Lattice lat,
-                                                      const uint32_t* __restrict__ key_sorted,
-                                                      const uint32_t* __restrict__ start,
-                                                      const uint4* __restrict__ table,
-                                                      const uint32_t* __restrict__ nchunks,
-                                                      uint32_t* key_out, uint32_t* count,
-                                                      double* forces, uint64_t* digest,
-                                                      uint32_t* recheck)
+__global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, StateF64 out,
+                                                    Lattice lat,
+                                                    const uint32_t* __restrict__ key_sorted,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint4* __restrict__ table,
+                                                    const uint32_t* __restrict__ nchunks,
+                                                    uint32_t* key_out, uint32_t* count,
+                                                    double* forces, uint64_t* digest,
+                                                    uint32_t* recheck)
 {
     __shared__ uint4 sl[kCapMixed];          // lattice x, y, z, sorted slot
     __shared__ uint2 sc[kCapMixed];          // top 16 bits of x|y, z: the coarse lattice
     __shared__ float svx[kCapMixed], svy[kCapMixed], svz[kCapMixed];
     __shared__ uint32_t sid[DIGEST ? kCapMixed : 1];
-    __shared__ uint16_t sq[kQueue * kBlock];
-    __shared__ Segment S;
+    __shared__ uint16_t sq[kQueue * kNbr];
+    __shared__ uint32_t wsum[17];
+    extern __shared__ __align__(16) unsigned char dyn[];
 
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= *nchunks) return;
     const uint4 ch = table[b];
     const int G = P.G;
+    Segment S = segment_view(dyn, G, wsum);
     const uint32_t row = ch.x, t0 = ch.y, t1 = ch.z;
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
     const uint32_t s = t0 + threadIdx.x;
@@ -735,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
         for (uint32_t base = 0; base < total; base += kCapMixed) {
             const uint32_t here = min((uint32_t)kCapMixed, total - base);
             __syncthreads();
-            for (uint32_t idx = threadIdx.x; idx < here; idx += kBlock) {
+            for (uint32_t idx = threadIdx.x; idx < here; idx += kNbr) {
                 const uint32_t g = base + idx;
                 const int k = seg_cell(S, nseg, g);
                 const uint32_t slot = S.segg[k] + (g - S.segc[k]);
@@ -798,13 +818,13 @@ __global__ __launch_bounds__(kBlock) void k_nbr_mixed(DevParams P, StateF64 in, 
             };
 #if BS_NBR_QUEUE
             auto drain = [&](uint32_t qn) {
-                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kBlock + threadIdx.x]);
+                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kNbr + threadIdx.x]);
             };
             uint32_t qn = 0;
             for (uint32_t k = kb; k < ke; ++k) {
                 const uint32_t kk = k - base;
                 if (coarse(kk) <= P.coarse_thr && k != selfk) {
-                    sq[qn * kBlock + threadIdx.x] = (uint16_t)kk;
+                    sq[qn * kNbr + threadIdx.x] = (uint16_t)kk;
                     ++qn;
                 }
                 if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
@@ -920,11 +940,12 @@ hipError_t launch_nbr_f64(const DevParams& P, StateF64 in, StateF64 out, const u
 {
     if (P.n == 0) return hipSuccess;
     const dim3 grid(round8(max_chunks));
+    const size_t dyn = segment_bytes(P.G);
     if (digest)
-        hipLaunchKernelGGL(k_nbr_f64<true>, grid, dim3(kBlock), 0, stream, P, in, out, key_sorted,
+        hipLaunchKernelGGL(k_nbr_f64<true>, grid, dim3(kNbr), dyn, stream, P, in, out, key_sorted,
                            start, table, nchunks, key_out, count, forces, digest);
     else
-        hipLaunchKernelGGL(k_nbr_f64<false>, grid, dim3(kBlock), 0, stream, P, in, out, key_sorted,
+        hipLaunchKernelGGL(k_nbr_f64<false>, grid, dim3(kNbr), dyn, stream, P, in, out, key_sorted,
                            start, table, nchunks, key_out, count, forces, digest);
     return hipGetLastError();
 }
@@ -937,11 +958,12 @@ hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Latti
 {
     if (P.n == 0) return hipSuccess;
     const dim3 grid(round8(max_chunks));
+    const size_t dyn = segment_bytes(P.G);
     if (digest)
-        hipLaunchKernelGGL(k_nbr_mixed<true>, grid, dim3(kBlock), 0, stream, P, in, out, lat,
+        hipLaunchKernelGGL(k_nbr_mixed<true>, grid, dim3(kNbr), dyn, stream, P, in, out, lat,
                            key_sorted, start, table, nchunks, key_out, count, forces, digest, recheck);
     else
-        hipLaunchKernelGGL(k_nbr_mixed<false>, grid, dim3(kBlock), 0, stream, P, in, out, lat,
+        hipLaunchKernelGGL(k_nbr_mixed<false>, grid, dim3(kNbr), dyn, stream, P, in, out, lat,
                            key_sorted, start, table, nchunks, key_out, count, forces, digest, recheck);
     return hipGetLastError();
 }
