@@ -876,3 +876,40 @@ extern "C" int bs_debug_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
     if (b) (void)hipFree(b);
     return rc;
 }
+
+// Flock metrics of the current state (see boids_metrics.hip).  Temporary
+// device memory is allocated and released inside the call.
+extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out)
+{
+    if (!e || !out) return BS_E_ARG;
+    if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    if (!(eps >= 0.0) || !isfinite(eps)) return fail(e, BS_E_ARG, "eps must be finite and >= 0");
+    const uint32_t n = e->prm.n;
+    memset(out, 0, sizeof(*out));
+    if (n == 0) return BS_OK;
+    const double F = e->prm.field;
+    // edge >= eps (links need only the 27 neighbouring cells); no more cells than ~8 per boid
+    double g = eps > 0.0 ? floor(F / (eps * 1.0001)) : 1024.0;
+    const double cap = std::max(1.0, 2.0 * floor(cbrt((double)n)));
+    g = std::min(std::min(g, cap), 1024.0);
+    const int G = std::max(1, (int)g);
+    if (e->mode == BS_MODE_ALLPAIRS_F32REF)
+        HIP_TRY(e, launch_f32_to_f64(n, e->fa, e->tmp_pos, e->tmp_vel, e->stream));
+    else
+        HIP_TRY(e, launch_unsort_f64(n, e->da, e->tmp_pos, e->tmp_vel, e->stream));
+    void* ws = nullptr;
+    const size_t bytes = metrics_workspace_bytes(n, G);
+    if (hipMalloc(&ws, bytes) != hipSuccess) return fail(e, BS_E_NOMEM, "metrics workspace allocation failed");
+    double* res = (double*)((char*)ws + bytes - 64);
+    hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos, e->tmp_vel, ws, res, e->stream);
+    double host[4] = {0, 0, 0, 0};
+    if (st == hipSuccess) st = hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    (void)hipFree(ws);
+    if (st != hipSuccess) return hip_fail(e, st, "metrics");
+    out->polarisation = host[0];
+    out->mean_nn = host[1];
+    out->clusters = (uint64_t)host[2];
+    out->largest = (uint64_t)host[3];
+    return BS_OK;
+}

@@ -71,7 +71,15 @@ EXPORTS = ["bs_create", "bs_destroy", "bs_last_error", "bs_generate_init", "bs_r
            "bs_set_state_f32", "bs_set_state_f64", "bs_step", "bs_sync", "bs_get_state_f32",
            "bs_get_state_f64", "bs_get_forces_f32", "bs_get_forces_f64", "bs_get_digest",
            "bs_get_info", "bs_get_timing", "bs_reset_timing", "bs_debug_thresholds",
-           "bs_debug_rechecks", "bs_debug_copy", "bs_version"]
+           "bs_debug_rechecks", "bs_debug_copy", "bs_metrics", "bs_version"]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [("polarisation", ctypes.c_double), ("mean_nn", ctypes.c_double),
+                ("clusters", ctypes.c_uint64), ("largest", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def lib():
@@ -106,6 +114,7 @@ def lib():
         "bs_debug_thresholds": (c_int, [P(Params), f64p]),
         "bs_debug_rechecks": (c_int, [vp, u64p]),
         "bs_debug_copy": (c_int, [ctypes.c_uint64, ctypes.c_int32, f64p]),
+        "bs_metrics": (c_int, [vp, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
         "bs_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -246,6 +255,13 @@ class Engine:
         out = ctypes.c_uint64()
         self._check(lib().bs_debug_rechecks(self._h, ctypes.byref(out)), "bs_debug_rechecks")
         return out.value
+
+    def metrics(self, eps, min_size=2) -> dict:
+        """Polarisation, mean nearest-neighbour distance, cluster count (graph
+        d < eps, components >= min_size) and largest cluster of the state."""
+        m = Metrics()
+        self._check(lib().bs_metrics(self._h, eps, min_size, ctypes.byref(m)), "bs_metrics")
+        return m.as_dict()
 
 
 class BoidSimulation:

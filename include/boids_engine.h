@@ -165,6 +165,18 @@ int bs_debug_thresholds(const bs_params* p, double* out6);
  * creation (guard-band hits).  0 for other modes. */
 int bs_debug_rechecks(bs_engine* e, uint64_t* out);
 
+/* Flock metrics of the current state, for long-run statistical comparison.
+ * Periodic distance as tools/calc_distance.pyx:6-14.  polarisation =
+ * |sum v/|v|| / N; mean_nn = mean nearest-neighbour distance; clusters =
+ * connected components of the graph d < eps with >= min_size members;
+ * largest = size of the biggest component.  No reference counterpart in the
+ * engine (the reference computes these offline in tools/). */
+typedef struct {
+    double polarisation, mean_nn;
+    uint64_t clusters, largest;
+} bs_metrics_out;
+int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out);
+
 /* Measurement helper: `reps` launches of a plain streaming copy of `bytes`
  * bytes on the current device (reference point for HBM byte counters).
  * *ms_per_copy receives the average HIP-event time per copy. */

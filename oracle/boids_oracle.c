@@ -192,3 +192,72 @@ int bo_init_reference(int kind, bo_rng* r, uint32_t n, double F,
     }
     return 0;
 }
+
+/* ---- flock metrics (brute force) ---------------------------------------- */
+
+static double bo_pdist(const double* a, const double* b, double F)
+{
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double d = fabs(a[k] - b[k]);
+        if (F - d < d) d = F - d;
+        s += d * d;
+    }
+    return sqrt(s);
+}
+
+static uint32_t bo_find(uint32_t* parent, uint32_t x)
+{
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+int bo_metrics(uint32_t n, double F, const double* pos, const double* vel, double eps,
+               uint32_t min_size, int nthreads, bo_metrics_out* out)
+{
+    if (!out || (n && (!pos || !vel))) return -1;
+    memset(out, 0, sizeof(*out));
+    if (n == 0) return 0;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    double sx = 0, sy = 0, sz = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* v = vel + 3 * (size_t)i;
+        double s = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (s > 0) { sx += v[0] / s; sy += v[1] / s; sz += v[2] / s; }
+    }
+    out->polarisation = sqrt(sx * sx + sy * sy + sz * sz) / n;
+    double* nn = (double*)malloc(sizeof(double) * n);
+    uint32_t* parent = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* size = (uint32_t*)calloc(n, sizeof(uint32_t));
+    if (!nn || !parent || !size) { free(nn); free(parent); free(size); return -1; }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        double best = INFINITY;
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == (uint32_t)ii) continue;
+            double d = bo_pdist(pos + 3 * (size_t)ii, pos + 3 * (size_t)j, F);
+            if (d < best) best = d;
+        }
+        nn[ii] = best;
+    }
+    double acc = 0;
+    for (uint32_t i = 0; i < n; ++i) acc += nn[i];
+    out->mean_nn = n > 1 ? acc / n : INFINITY;
+    for (uint32_t i = 0; i < n; ++i) parent[i] = i;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = i + 1; j < n; ++j)
+            if (bo_pdist(pos + 3 * (size_t)i, pos + 3 * (size_t)j, F) < eps) {
+                uint32_t a = bo_find(parent, i), b = bo_find(parent, j);
+                if (a != b) parent[a > b ? a : b] = a < b ? a : b;
+            }
+    for (uint32_t i = 0; i < n; ++i) size[bo_find(parent, i)]++;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (size[i] >= min_size && size[i] > 0) out->clusters++;
+        if (size[i] > out->largest) out->largest = size[i];
+    }
+    free(nn); free(parent); free(size);
+    return 0;
+}

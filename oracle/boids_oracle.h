@@ -86,6 +86,21 @@ double bo_drand48(bo_rng* r);
 int bo_init_reference(int kind, bo_rng* r, uint32_t n, double field,
                       double vmax, double vmin, float* pos, float* vel);
 
+/* Flock metrics for long-run statistical comparisons.  Distances use the
+ * periodic metric of tools/calc_distance.pyx:6-14 (per axis
+ * min(|dx|, F - |dx|)).
+ *   polarisation = |sum_i v_i/|v_i|| / N   (boids with |v| = 0 add nothing)
+ *   mean_nn      = mean_i min_{j != i} d(i, j)
+ *   clusters     = connected components of the graph d(i, j) < eps with at
+ *                  least min_size members; largest = size of the biggest.
+ * Brute force O(N^2) (OpenMP). */
+typedef struct {
+    double polarisation, mean_nn;
+    uint64_t clusters, largest;
+} bo_metrics_out;
+int bo_metrics(uint32_t n, double field, const double* pos, const double* vel, double eps,
+               uint32_t min_size, int nthreads, bo_metrics_out* out);
+
 #ifdef __cplusplus
 }
 #endif

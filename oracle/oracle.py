@@ -63,8 +63,28 @@ def lib():
         L.bo_init_reference.argtypes = [ctypes.c_int, P(Rng), ctypes.c_uint32, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, P(ctypes.c_float),
                                         P(ctypes.c_float)]
+        L.bo_metrics.restype = ctypes.c_int
+        L.bo_metrics.argtypes = [ctypes.c_uint32, ctypes.c_double, P(ctypes.c_double), P(ctypes.c_double),
+                                 ctypes.c_double, ctypes.c_uint32, ctypes.c_int, P(MetricsOut)]
         _lib = L
     return _lib
+
+
+class MetricsOut(ctypes.Structure):
+    _fields_ = [("polarisation", ctypes.c_double), ("mean_nn", ctypes.c_double),
+                ("clusters", ctypes.c_uint64), ("largest", ctypes.c_uint64)]
+
+
+def metrics(field, pos, vel, eps, min_size=2, nthreads=0) -> dict:
+    """Brute-force flock metrics (see bo_metrics in boids_oracle.h)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    vel = np.ascontiguousarray(vel, np.float64)
+    out = MetricsOut()
+    rc = lib().bo_metrics(pos.shape[0], field, _ptr(pos, ctypes.c_double), _ptr(vel, ctypes.c_double),
+                          eps, min_size, nthreads, ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError("oracle metrics failed")
+    return {k: getattr(out, k) for k, _ in out._fields_}
 
 
 def params(field, sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008),

@@ -262,3 +262,42 @@ def test_small_field_falls_back_to_allpairs():
         O.step(P, p, v)
     gp, gv = e.state_f64()
     assert np.array_equal(gp, p) and np.array_equal(gv, v)
+
+
+# -------------------------------------------------------------- metrics ---
+@pytest.mark.parametrize("mode", [E.MODE_ALLPAIRS_F32REF, E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+def test_metrics_match_oracle(mode):
+    n, F = 6000, 0.72
+    st = iso_state(n, F, 7)
+    e = make(n, F, CFG0, mode)
+    e.set_state(xyzv6=st)
+    e.step(20)
+    p, v = e.state_f64()
+    for eps, ms in ((0.02, 2), (0.05, 1), (0.004, 2)):
+        g = e.metrics(eps, ms)
+        o = O.metrics(F, p, v, eps, ms)
+        assert g["clusters"] == o["clusters"] and g["largest"] == o["largest"], (eps, g, o)
+        assert abs(g["polarisation"] - o["polarisation"]) < 1e-12
+        assert abs(g["mean_nn"] - o["mean_nn"]) < 1e-12 * F
+
+
+def test_long_run_statistics_fp64_vs_mixed_vs_oracle():
+    # Chaotic runs diverge pointwise; their statistics must agree.  N = 2^20,
+    # 600 steps on the GPU in both precisions; time-averaged over the last
+    # 100 steps (every 20).
+    n, F = 1 << 20, 4.0
+    st = iso_state(n, F, 2)
+    avg = {}
+    for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
+        e = make(n, F, CFG0, mode)
+        e.set_state(xyzv6=st)
+        e.step(500)
+        acc = []
+        for _ in range(5):
+            e.step(20)
+            acc.append(e.metrics(0.02, 2))
+        avg[mode] = {k: np.mean([a[k] for a in acc]) for k in acc[0]}
+        say(E.MODE_NAMES[mode], avg[mode])
+    a, b = avg[E.MODE_GRID_F64], avg[E.MODE_GRID_MIXED]
+    for k in ("polarisation", "mean_nn", "clusters"):
+        assert abs(a[k] - b[k]) <= 0.01 * abs(a[k]), (k, a[k], b[k])

@@ -295,3 +295,30 @@ def test_digest_is_a_neighbour_set_hash():
         assert counts(d[i]) == (len(nb[0]), len(nb[1]), len(nb[2]))
         for k in range(3):
             assert int(d[i, 1 + k]) == sum(O.mix64(j) for j in nb[k]) % 2**64
+
+
+# ------------------------------------------------------------- metrics ---
+def test_metrics_closed_forms():
+    # polarisation: aligned -> 1, opposed pairs -> 0; zero velocities add nothing
+    pos = np.array([[0.1, 0.1, 0.1], [0.5, 0.5, 0.5], [0.9, 0.2, 0.4], [0.3, 0.7, 0.6]])
+    vel = np.tile([0.0, 0.003, 0.0], (4, 1))
+    assert abs(O.metrics(1.0, pos, vel, 0.01)["polarisation"] - 1.0) < 1e-15
+    vel[1] *= -1; vel[3] *= -1
+    assert O.metrics(1.0, pos, vel, 0.01)["polarisation"] < 1e-15
+    vel[:] = 0
+    assert O.metrics(1.0, pos, vel, 0.01)["polarisation"] == 0.0
+    # nearest neighbour on a cubic lattice of spacing 0.1 (periodic: every
+    # point has six neighbours at 0.1, also across the boundary)
+    g = (np.arange(10) + 0.5) / 10
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    m = O.metrics(1.0, lat, np.zeros_like(lat), 0.1001, 1)
+    assert abs(m["mean_nn"] - 0.1) < 1e-12
+    assert m["clusters"] == 1 and m["largest"] == 1000          # eps > spacing: one component
+    m = O.metrics(1.0, lat, np.zeros_like(lat), 0.0999, 1)
+    assert m["clusters"] == 1000 and m["largest"] == 1
+    assert O.metrics(1.0, lat, np.zeros_like(lat), 0.0999, 2)["clusters"] == 0
+    # periodic distance across the wall, and chains
+    pos = np.array([[0.005, 0.5, 0.5], [0.995, 0.5, 0.5], [0.5, 0.5, 0.5], [0.51, 0.5, 0.5], [0.52, 0.5, 0.5]])
+    m = O.metrics(1.0, pos, np.ones_like(pos), 0.0105, 2)
+    assert m["clusters"] == 2 and m["largest"] == 3
+    assert abs(m["mean_nn"] - 0.01) < 1e-12
