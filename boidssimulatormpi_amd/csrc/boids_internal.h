@@ -16,8 +16,10 @@ constexpr int kBlock = 256;        // threads per block of the streaming kernels
 #endif
 constexpr int kNbr = BS_NBR_BLOCK;  // threads (= boids per chunk) of the neighbour kernels
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
-constexpr int kCapF64 = 384;       // LDS candidate capacity per pass (FP64 kernel)
-constexpr int kCapMixed = 512;     // LDS candidate capacity per pass (mixed kernel)
+// LDS candidate capacity per pass; a row segment of a kNbr-boid chunk holds
+// about kNbr + 2 cells of candidates, larger segments take several passes.
+constexpr int kCapF64 = kNbr + 128;
+constexpr int kCapMixed = kNbr + 256;
 #ifndef BS_QUEUE_DEPTH
 #define BS_QUEUE_DEPTH 16
 #endif
@@ -25,6 +27,14 @@ constexpr int kCapMixed = 512;     // LDS candidate capacity per pass (mixed ker
 #define BS_NBR_QUEUE 1             // 1: filter -> per-lane queue -> dense pass; 0: evaluate inline
 #endif
 constexpr int kQueue = BS_QUEUE_DEPTH;   // per-lane queue of in-range candidates (LDS indices)
+#ifndef BS_NBR_WAVES
+#define BS_NBR_WAVES 0             // >0: ask the compiler for at least this many waves per SIMD
+#endif
+#if BS_NBR_WAVES > 0
+#define BS_NBR_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BS_NBR_WAVES)))
+#else
+#define BS_NBR_OCCUPANCY
+#endif
 
 // Everything a kernel needs about the rules, by value in the kernarg segment.
 struct DevParams {

@@ -480,7 +480,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t grid)
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kNbr) void k_nbr_f64(DevParams P, StateF64 in, StateF64 out,
+__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, StateF64 in, StateF64 out,
                                                   const uint32_t* __restrict__ key_sorted,
                                                   const uint32_t* __restrict__ start,
                                                   const uint4* __restrict__ table,
@@ -557,31 +557,30 @@ __global__ __launch_bounds__(kNbr) void k_nbr_f64(DevParams P, StateF64 in, Stat
             __syncthreads();
             if (!active) continue;
             // Full rule evaluation of one candidate inside the largest radius.
+            // The decisions are the oracle's expressions; the sums are
+            // branch-free (adding +0.0 leaves a sum unchanged) so that
+            // consecutive pairs overlap.
             auto pair = [&](uint32_t kk) {
                     const double qx = sx[kk], qy = sy[kk], qz = sz[kk];
+                    const double wx = svx[kk], wy = svy[kk], wz = svz[kk];
                     const double ex = px - qx, ey = py - qy, ez = pz - qz;
                     const double s2 = ex * ex + ey * ey + ez * ez;
-                    bool in_c = s2 < P.s_thr[COH], in_s = s2 < P.s_thr[SEP], in_a = s2 < P.s_thr[ALI];
-                    if ((in_c && k_c) || (in_s && k_s) || (in_a && k_a)) {
-                        const double gx = qx - px, gy = qy - py, gz = qz - pz;
-                        const double dot = vx * gx + vy * gy + vz * gz;
-                        const double c = dot / (vn * sqrt(s2));
-                        if (k_c) in_c = in_c && c >= P.c_thr[COH] && c <= 1.0;
-                        if (k_s) in_s = in_s && c >= P.c_thr[SEP] && c <= 1.0;
-                        if (k_a) in_a = in_a && c >= P.c_thr[ALI] && c <= 1.0;
-                    }
-                    if (in_c) {
-                        nc++; scx += qx; scy += qy; scz += qz;
-                        if (DIGEST) hc += mix64(sid[kk]);
-                    }
+                    const double gx = qx - px, gy = qy - py, gz = qz - pz;
+                    const double dot = vx * gx + vy * gy + vz * gz;
+                    const double t = sqrt(s2);
+                    const double c = dot / (vn * t);
+                    const bool cin = c <= 1.0;
+                    const bool in_c = s2 < P.s_thr[COH] && (!k_c || (c >= P.c_thr[COH] && cin));
+                    const bool in_s = s2 < P.s_thr[SEP] && (!k_s || (c >= P.c_thr[SEP] && cin));
+                    const bool in_a = s2 < P.s_thr[ALI] && (!k_a || (c >= P.c_thr[ALI] && cin));
+                    nc += in_c; scx += in_c ? qx : 0.0; scy += in_c ? qy : 0.0; scz += in_c ? qz : 0.0;
+                    na += in_a; sax += in_a ? wx : 0.0; say += in_a ? wy : 0.0; saz += in_a ? wz : 0.0;
                     if (in_s) {
-                        const double t = sqrt(s2);
                         ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t;
-                        if (DIGEST) hs += mix64(sid[kk]);
                     }
-                    if (in_a) {
-                        na++; sax += svx[kk]; say += svy[kk]; saz += svz[kk];
-                        if (DIGEST) ha += mix64(sid[kk]);
+                    if (DIGEST) {
+                        const uint64_t h = mix64(sid[kk]);
+                        hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
                     }
             };
             // Phase 1: squared distance only (any rounding may be used here:
@@ -590,19 +589,38 @@ __global__ __launch_bounds__(kNbr) void k_nbr_f64(DevParams P, StateF64 in, Stat
 #if BS_NBR_QUEUE
             // Candidates inside the largest radius go to this lane's queue,
             // drained (phase 2, densely packed) when any lane fills up.
+            // Four candidates per iteration keep four LDS->distance chains in
+            // flight; a slot is always written and kept only on a hit.
             auto drain = [&](uint32_t qn) {
-                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kNbr + threadIdx.x]);
+                uint32_t j = 0;
+                for (; j + 2 <= qn; j += 2) {
+                    const uint32_t k0 = sq[j * kNbr + threadIdx.x], k1 = sq[(j + 1) * kNbr + threadIdx.x];
+                    pair(k0);
+                    pair(k1);
+                }
+                if (j < qn) pair(sq[j * kNbr + threadIdx.x]);
             };
-            uint32_t qn = 0;
-            for (uint32_t k = kb; k < ke; ++k) {
+            auto near = [&](uint32_t k) {
                 const uint32_t kk = k - base;
                 const double ex = px - sx[kk], ey = py - sy[kk], ez = pz - sz[kk];
                 const double s2 = fma(ex, ex, fma(ey, ey, ez * ez));
-                if (s2 < s_filter && k != selfk) {
-                    sq[qn * kNbr + threadIdx.x] = (uint16_t)kk;
-                    ++qn;
-                }
+                return s2 < s_filter && k != selfk;
+            };
+            uint32_t qn = 0;
+            uint32_t k = kb;
+            for (; k + 4 <= ke; k += 4) {
+                if (__any(qn > (uint32_t)kQueue - 4)) { drain(qn); qn = 0; }
+                const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
+                const uint32_t kk = k - base;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)kk; qn += h0;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+            }
+            for (; k < ke; ++k) {
                 if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(k - base);
+                qn += near(k);
             }
             drain(qn);
 #else
@@ -681,7 +699,7 @@ __device__ __forceinline__ int rule_verdict(const DevParams& P, int r, float s, 
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, StateF64 out,
+__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P, StateF64 in, StateF64 out,
                                                     Lattice lat,
                                                     const uint32_t* __restrict__ key_sorted,
                                                     const uint32_t* __restrict__ start,
@@ -734,7 +752,9 @@ __global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, St
     // a vanishing FP32 velocity makes the FP32 cosine meaningless: recheck
     const bool vbad = !(fvv > 1e-30f);
     const float inv_vn = vbad ? 0.f : __builtin_amdgcn_rsqf(fvv);
-    float scx = 0, scy = 0, scz = 0, ccx = 0, ccy = 0, ccz = 0;
+    // cohesion: exact integer sum of lattice displacements; separation and
+    // alignment: FP32 with Kahan compensation
+    int64_t scx = 0, scy = 0, scz = 0;
     float ssx = 0, ssy = 0, ssz = 0, csx = 0, csy = 0, csz = 0;
     float sax = 0, say = 0, saz = 0, cax = 0, cay = 0, caz = 0;
     int nc = 0, ns = 0, na = 0;
@@ -770,39 +790,33 @@ __global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, St
             // FP32 rule evaluation of one coarse-filtered candidate.
             auto pair = [&](uint32_t kk) {
                 const uint4 q = sl[kk];
-                const float gx = (float)(int32_t)(q.x - me.x);
-                const float gy = (float)(int32_t)(q.y - me.y);
-                const float gz = (float)(int32_t)(q.z - me.z);
+                const float wx = svx[kk], wy = svy[kk], wz = svz[kk];
+                const int32_t ix = (int32_t)(q.x - me.x), iy = (int32_t)(q.y - me.y), iz = (int32_t)(q.z - me.z);
+                const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
                 const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
-                if (!(s2 < P.s_hi_max)) return;
                 const float inv_d = __builtin_amdgcn_rsqf(s2);
                 const float c = fmaf(fvz, gz, fmaf(fvy, gy, fvx * gx)) * inv_d * inv_vn;
                 const float band = P.band_k * inv_d;
-                int vc = rule_verdict(P, COH, s2, c, band);
-                int vs = rule_verdict(P, SEP, s2, c, band);
-                int va = rule_verdict(P, ALI, s2, c, band);
-                bool in_c, in_s, in_a;
+                const int vc = rule_verdict(P, COH, s2, c, band);
+                const int vs = rule_verdict(P, SEP, s2, c, band);
+                const int va = rule_verdict(P, ALI, s2, c, band);
+                bool in_c = vc > 0, in_s = vs > 0, in_a = va > 0;
                 if (vc < 0 || vs < 0 || va < 0 || (vbad && (vc | vs | va))) {
                     const uint32_t j = q.w;
                     exact_decide(P, px, py, pz, vx, vy, vz, vn, in.x[j], in.y[j], in.z[j], in_s, in_a, in_c);
                     ++nre;
-                } else {
-                    in_c = vc > 0; in_s = vs > 0; in_a = va > 0;
                 }
-                if (in_c) {
-                    nc++;
-                    kahan_add(scx, ccx, gx); kahan_add(scy, ccy, gy); kahan_add(scz, ccz, gz);
-                    if (DIGEST) hc += mix64(sid[kk]);
-                }
-                if (in_s) {
-                    ns++;
-                    kahan_add(ssx, csx, -gx * inv_d); kahan_add(ssy, csy, -gy * inv_d); kahan_add(ssz, csz, -gz * inv_d);
-                    if (DIGEST) hs += mix64(sid[kk]);
-                }
-                if (in_a) {
-                    na++;
-                    kahan_add(sax, cax, svx[kk]); kahan_add(say, cay, svy[kk]); kahan_add(saz, caz, svz[kk]);
-                    if (DIGEST) ha += mix64(sid[kk]);
+                nc += in_c;
+                scx += in_c ? (int64_t)ix : 0; scy += in_c ? (int64_t)iy : 0; scz += in_c ? (int64_t)iz : 0;
+                ns += in_s;
+                kahan_add(ssx, csx, in_s ? -gx * inv_d : 0.f);
+                kahan_add(ssy, csy, in_s ? -gy * inv_d : 0.f);
+                kahan_add(ssz, csz, in_s ? -gz * inv_d : 0.f);
+                na += in_a;
+                kahan_add(sax, cax, in_a ? wx : 0.f); kahan_add(say, cay, in_a ? wy : 0.f); kahan_add(saz, caz, in_a ? wz : 0.f);
+                if (DIGEST) {
+                    const uint64_t h = mix64(sid[kk]);
+                    hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
                 }
             };
             // Phase 1: coarse squared distance on the top 16 lattice bits
@@ -818,16 +832,30 @@ __global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, St
             };
 #if BS_NBR_QUEUE
             auto drain = [&](uint32_t qn) {
-                for (uint32_t j = 0; j < qn; ++j) pair(sq[j * kNbr + threadIdx.x]);
-            };
-            uint32_t qn = 0;
-            for (uint32_t k = kb; k < ke; ++k) {
-                const uint32_t kk = k - base;
-                if (coarse(kk) <= P.coarse_thr && k != selfk) {
-                    sq[qn * kNbr + threadIdx.x] = (uint16_t)kk;
-                    ++qn;
+                uint32_t j = 0;
+                for (; j + 2 <= qn; j += 2) {
+                    const uint32_t k0 = sq[j * kNbr + threadIdx.x], k1 = sq[(j + 1) * kNbr + threadIdx.x];
+                    pair(k0);
+                    pair(k1);
                 }
+                if (j < qn) pair(sq[j * kNbr + threadIdx.x]);
+            };
+            auto near = [&](uint32_t k) { return coarse(k - base) <= P.coarse_thr && k != selfk; };
+            uint32_t qn = 0;
+            uint32_t k = kb;
+            for (; k + 4 <= ke; k += 4) {
+                if (__any(qn > (uint32_t)kQueue - 4)) { drain(qn); qn = 0; }
+                const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
+                const uint32_t kk = k - base;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)kk; qn += h0;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+            }
+            for (; k < ke; ++k) {
                 if (__any(qn == (uint32_t)kQueue)) { drain(qn); qn = 0; }
+                sq[qn * kNbr + threadIdx.x] = (uint16_t)(k - base);
+                qn += near(k);
             }
             drain(qn);
 #else
@@ -844,7 +872,7 @@ __global__ __launch_bounds__(kNbr) void k_nbr_mixed(DevParams P, StateF64 in, St
     double dcx = 0, dcy = 0, dcz = 0, dax = 0, day = 0, daz = 0;
     if (nc != 0) {
         const double k = P.unit / (double)nc;
-        dcx = ((double)scx - (double)ccx) * k; dcy = ((double)scy - (double)ccy) * k; dcz = ((double)scz - (double)ccz) * k;
+        dcx = (double)scx * k; dcy = (double)scy * k; dcz = (double)scz * k;
     }
     if (na != 0) {
         const double k = (double)na;
