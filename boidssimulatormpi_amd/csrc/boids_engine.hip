@@ -900,8 +900,8 @@ extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metric
     void* ws = nullptr;
     const size_t bytes = metrics_workspace_bytes(n, G);
     if (hipMalloc(&ws, bytes) != hipSuccess) return fail(e, BS_E_NOMEM, "metrics workspace allocation failed");
-    double* res = (double*)((char*)ws + bytes - 64);
-    hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos, e->tmp_vel, ws, res, e->stream);
+    double* res = nullptr;
+    hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos, e->tmp_vel, ws, bytes, &res, e->stream);
     double host[4] = {0, 0, 0, 0};
     if (st == hipSuccess) st = hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, e->stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e->stream);

@@ -110,8 +110,11 @@ hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Latti
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
 // flock metrics (boids_metrics.hip)
 size_t metrics_workspace_bytes(uint32_t n, int G);
+// Workspace regions are carved from `ws` (metrics_workspace_bytes long);
+// *out4_dev receives the device address of the 4 result doubles.
 hipError_t launch_metrics(uint32_t n, double F, int G, double eps, uint32_t min_size, const double* pos3,
-                          const double* vel3, void* ws, double* out5_dev, hipStream_t stream);
+                          const double* vel3, void* ws, size_t ws_bytes, double** out4_dev,
+                          hipStream_t stream);
 hipError_t launch_f32_to_f64(uint32_t n, StateF32 st, double* pos3, double* vel3, hipStream_t stream);
 hipError_t launch_unsort_f64(uint32_t n, StateF64 src, double* pos3, double* vel3,
                              hipStream_t stream);

@@ -236,37 +236,67 @@ hipError_t launch_f32_to_f64(uint32_t n, StateF32 st, double* pos3, double* vel3
     return hipGetLastError();
 }
 
-// Host orchestration.  All temporaries come from `ws` (see metrics_workspace_bytes).
-size_t metrics_workspace_bytes(uint32_t n, int G)
+// Workspace layout: one function carves the regions, both for sizing
+// (base == nullptr) and for the launch, so the two cannot disagree.
+struct MetricsWs {
+    double4* cpos;      // n    cell-ordered positions (id bits in w)
+    double* nn;         // n
+    uint32_t *key, *perm, *permid, *ids, *parent, *size;   // n each
+    uint32_t *count, *start;                               // cells each
+    uint32_t* partials; // scan tiles
+    double* out;        // 4
+    size_t bytes;       // total, from the (unaligned) base
+};
+
+static_assert(sizeof(double4) == 32, "double4 is 32 bytes");
+
+static MetricsWs metrics_layout(char* base, uint32_t n, int G)
 {
     const size_t cells = (size_t)G * G * G + 1;
-    return 16 * (size_t)n            // cpos
-         + 8 * (size_t)n             // nn
-         + 4 * 6 * (size_t)n         // key, perm, permid, ids, parent, size
-         + 4 * 2 * cells             // count, start
-         + 4 * (cells / 4096 + 2)    // scan partials
-         + 8 * 8 + 256;              // out + alignment slack
+    const size_t ntiles = (cells + 4095) / 4096;
+    MetricsWs m;
+    uintptr_t p = ((uintptr_t)base + 255) & ~(uintptr_t)255;
+    auto take = [&](size_t bytes) {
+        const uintptr_t at = p;
+        p = (p + bytes + 255) & ~(uintptr_t)255;
+        return at;
+    };
+    m.cpos = (double4*)take(sizeof(double4) * (size_t)n);
+    m.nn = (double*)take(sizeof(double) * (size_t)n);
+    m.key = (uint32_t*)take(4 * (size_t)n);
+    m.perm = (uint32_t*)take(4 * (size_t)n);
+    m.permid = (uint32_t*)take(4 * (size_t)n);
+    m.ids = (uint32_t*)take(4 * (size_t)n);
+    m.parent = (uint32_t*)take(4 * (size_t)n);
+    m.size = (uint32_t*)take(4 * (size_t)n);
+    m.count = (uint32_t*)take(4 * cells);
+    m.start = (uint32_t*)take(4 * cells);
+    m.partials = (uint32_t*)take(4 * ntiles);
+    m.out = (double*)take(sizeof(double) * 4);
+    m.bytes = (size_t)(p - (uintptr_t)base);
+    return m;
+}
+
+size_t metrics_workspace_bytes(uint32_t n, int G)
+{
+    return metrics_layout(nullptr, n, G).bytes + 256;   // + worst-case alignment of a real base
 }
 
 hipError_t launch_metrics(uint32_t n, double F, int G, double eps, uint32_t min_size, const double* pos3,
-                          const double* vel3, void* ws, double* out5_dev, hipStream_t stream)
+                          const double* vel3, void* ws, size_t ws_bytes, double** out4_dev, hipStream_t stream)
 {
+    const MetricsWs m = metrics_layout((char*)ws, n, G);
+    if ((char*)(m.out + 4) > (char*)ws + ws_bytes) return hipErrorInvalidValue;   // layout must fit
+    *out4_dev = m.out;
     if (n == 0) return hipSuccess;
     MGrid g;
     g.G = G; g.F = F; g.edge = F / G; g.inv_edge = G / F; g.n = n;
     const uint32_t cells = (uint32_t)((size_t)G * G * G + 1);
-    char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
-    double4* cpos = (double4*)w; w += 16 * (size_t)n;
-    double* nn = (double*)w; w += 8 * (size_t)n;
-    uint32_t* key = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* perm = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* permid = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* ids = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* parent = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* size = (uint32_t*)w; w += 4 * (size_t)n;
-    uint32_t* count = (uint32_t*)w; w += 4 * (size_t)cells;
-    uint32_t* start = (uint32_t*)w; w += 4 * (size_t)cells;
-    uint32_t* partials = (uint32_t*)w;
+    double4* cpos = m.cpos;
+    double* nn = m.nn;
+    uint32_t *key = m.key, *perm = m.perm, *permid = m.permid, *ids = m.ids, *parent = m.parent, *size = m.size;
+    uint32_t *count = m.count, *start = m.start, *partials = m.partials;
+    double* out5_dev = m.out;
     hipLaunchKernelGGL(k_m_fill, dim3(mblocks(cells)), dim3(kBlock), 0, stream, count, cells, 0u);
     hipLaunchKernelGGL(k_m_hash, dim3(mblocks(n)), dim3(kBlock), 0, stream, g, pos3, key, count);
     hipError_t st = launch_scan(count, start, cells, partials, stream);
