@@ -41,6 +41,17 @@ struct bs_engine {
     uint32_t *partials = nullptr, *nchunks = nullptr, *perm = nullptr, *permid = nullptr;
     uint4* table = nullptr;
     uint32_t max_chunks = 0, ncell1 = 0;
+    // flock table (one entry for a single-flock engine)
+    std::vector<FlockDesc> flocks;
+    std::vector<double> flock_field;
+    FlockDesc* flocks_dev = nullptr;
+    uint32_t* flock_of = nullptr;      // slot -> flock (batches only)
+    uint32_t rows = 0;
+    int max_g = 0;
+    bool batch = false;
+    // one captured step (BS_FLAG_GRAPH)
+    hipGraphExec_t graph = nullptr;
+    bool capturing = false;
     Lattice lat{};
     uint32_t* recheck = nullptr;
     // outputs
@@ -108,7 +119,7 @@ struct Timed {
     hipEvent_t a = nullptr;
     Timed(bs_engine* e_, int k) : e(e_), kernel(k)
     {
-        if (e->prm.flags & BS_FLAG_TIMING) {
+        if ((e->prm.flags & BS_FLAG_TIMING) && !e->capturing) {
             a = take_event(e);
             if (a) (void)hipEventRecord(a, e->stream);
         }
@@ -439,6 +450,7 @@ static void release(bs_engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto& p : e->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto ev : e->free_events) (void)hipEventDestroy(ev);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -463,36 +475,94 @@ static int alloc_f32(bs_engine* e, StateF32* s, uint32_t n)
     return BS_OK;
 }
 
-extern "C" int bs_create(const bs_params* p, bs_engine** out)
+static bool is_grid(int mode) { return mode == BS_MODE_GRID_F64 || mode == BS_MODE_GRID_MIXED; }
+
+static Flocks flock_view(const bs_engine* e)
 {
-    if (!out) return fail(nullptr, BS_E_ARG, "bs_create: out is NULL");
-    *out = nullptr;
-    if (!p) return fail(nullptr, BS_E_ARG, "bs_create: params is NULL");
+    Flocks F;
+    F.fl = e->flocks_dev;
+    F.flock_of = e->flock_of;
+    F.count = (uint32_t)e->flocks.size();
+    F.n = e->prm.n;
+    F.rows = e->rows;
+    return F;
+}
+
+// Shared by bs_create (one flock) and bs_create_batch (a sweep): validates
+// every flock, derives its thresholds and grid, lays the flocks out
+// back to back (boids, cells and rows) and allocates the device buffers.
+static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_engine** out)
+{
+    const bs_params* p = &fp[0];
     if (!(p->field > 0.0) || !isfinite(p->field)) return fail(nullptr, BS_E_ARG, "field size must be positive and finite");
     if (p->mode < BS_MODE_ALLPAIRS_F32REF || p->mode > BS_MODE_GRID_MIXED) return fail(nullptr, BS_E_ARG, "unknown mode");
-    if (p->n > (1u << 28)) return fail(nullptr, BS_E_ARG, "n too large (max 2^28)");
+    if (batch && !is_grid(p->mode)) return fail(nullptr, BS_E_ARG, "bs_create_batch: grid modes only");
+    uint64_t ntot = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const bs_params& q = fp[k];
+        const std::string at = batch ? "flock " + std::to_string(k) + ": " : std::string();
+        if (!(q.field > 0.0) || !isfinite(q.field)) return fail(nullptr, BS_E_ARG, at + "field size must be positive and finite");
+        if (q.mode != p->mode || q.device != p->device || q.flags != p->flags)
+            return fail(nullptr, BS_E_ARG, at + "mode, device and flags must be the same for every flock");
+        ntot += q.n;
+    }
+    if (ntot > (1u << 28)) return fail(nullptr, BS_E_ARG, "n too large (max 2^28 boids per engine)");
 
     bs_engine* e = new bs_engine();
     e->prm = *p;
+    e->prm.n = (uint32_t)ntot;
     e->mode = p->mode;
+    e->batch = batch;
     std::string why;
     int rc = derive_thresholds(p, &e->P, &why);
     if (rc) { g_create_error = why; release(e); return rc; }
     e->P.n = p->n;
 
-    if (e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED) {
-        int G = cells_per_axis(p);
-        if (G < 0) { release(e); return fail(nullptr, BS_E_ARG, "cells_per_axis must be >= 3 with edge >= max radius"); }
-        if (G < 3) {
-            e->mode = BS_MODE_ALLPAIRS_F64;   // fewer than 3 cells per axis: 27-cell windows would overlap
+    if (is_grid(e->mode)) {
+        uint64_t cells = 0, rows = 0;
+        bool coarse = false;
+        for (uint32_t k = 0; k < count; ++k) {
+            const bs_params& q = fp[k];
+            const std::string at = batch ? "flock " + std::to_string(k) + ": " : std::string();
+            FlockDesc d{};
+            rc = derive_thresholds(&q, &d.P, &why);
+            if (rc) { release(e); return fail(nullptr, rc, at + why); }
+            d.P.n = q.n;
+            const int G = cells_per_axis(&q);
+            if (G < 0) { release(e); return fail(nullptr, BS_E_ARG, at + "cells_per_axis must be >= 3 with edge >= max radius"); }
+            if (G < 3) {
+                if (batch) { release(e); return fail(nullptr, BS_E_ARG, at + "fewer than 3 cells per axis (field too small for its radii); step it in its own engine"); }
+                e->mode = BS_MODE_ALLPAIRS_F64;   // 27-cell windows would overlap
+                break;
+            }
+            d.P.G = G;
+            d.P.inv_edge = G / q.field;
+            d.cell_base = (uint32_t)cells;
+            d.row_base = (uint32_t)rows;
+            d.nrows = (uint32_t)G * (uint32_t)G;
+            cells += (uint64_t)G * G * G;
+            rows += (uint64_t)G * G;
+            if (!(d.P.unit * 8.0 < 1e-6)) coarse = true;   // lattice too coarse for the mixed guard bands
+            e->flocks.push_back(d);
+            e->flock_field.push_back(q.field);
+            e->max_g = std::max(e->max_g, G);
+        }
+        if (e->mode != BS_MODE_ALLPAIRS_F64) {
+            if (cells + 1 > 0x7fffffffull) { release(e); return fail(nullptr, BS_E_ARG, "too many grid cells over all flocks"); }
+            uint32_t base = 0;
+            for (uint32_t k = 0; k < count; ++k) { e->flocks[k].boid_base = base; base += fp[k].n; }
+            e->rows = (uint32_t)rows;
+            e->ncell1 = (uint32_t)(cells + 1);
+            e->G = e->flocks[0].P.G;
+            e->edge = fp[0].field / e->G;
+            e->P.G = e->G;
+            e->P.inv_edge = e->flocks[0].P.inv_edge;
+            if (e->mode == BS_MODE_GRID_MIXED && coarse) e->mode = BS_MODE_GRID_F64;
         } else {
-            e->G = G;
-            e->edge = p->field / G;
-            e->P.G = G;
-            e->P.inv_edge = G / p->field;
+            e->flocks.clear();
+            e->flock_field.clear();
         }
     }
-    if (e->mode == BS_MODE_GRID_MIXED && !(e->P.unit * 8.0 < 1e-6)) e->mode = BS_MODE_GRID_F64;   // lattice too coarse
 
     hipError_t st = hipSetDevice(p->device);
     if (st != hipSuccess) { std::string m = std::string("hipSetDevice: ") + hipGetErrorString(st); release(e); g_create_error = m; return BS_E_HIP; }
@@ -500,7 +570,7 @@ extern "C" int bs_create(const bs_params* p, bs_engine** out)
     st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (st != hipSuccess) { std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(st); e->stream = nullptr; release(e); g_create_error = m; return BS_E_HIP; }
 
-    const uint32_t n = p->n;
+    const uint32_t n = e->prm.n;
     auto bail = [&](int code) { std::string m = e->err; release(e); g_create_error = m; return code; };
     if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
         if ((rc = alloc_f32(e, &e->fa, n)) || (rc = alloc_f32(e, &e->fb, n))) return bail(rc);
@@ -512,18 +582,27 @@ extern "C" int bs_create(const bs_params* p, bs_engine** out)
     }
     if ((rc = dev_alloc(e, &e->tmp_pos, 3 * (size_t)n)) || (rc = dev_alloc(e, &e->tmp_vel, 3 * (size_t)n))) return bail(rc);
     if (p->flags & BS_FLAG_DIGEST) { if ((rc = dev_alloc(e, &e->digest, 4 * (size_t)n))) return bail(rc); }
-    if (e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED) {
-        const uint64_t G = (uint64_t)e->G;
-        e->ncell1 = (uint32_t)(G * G * G + 1);
+    if (is_grid(e->mode)) {
         // sum over rows of ceil(n_row / kNbr) <= n / kNbr + rows
-        e->max_chunks = (n + kNbr - 1) / kNbr + (uint32_t)(G * G) + 1;
+        e->max_chunks = (n + kNbr - 1) / kNbr + e->rows + 1;
         const uint32_t ntiles = (e->ncell1 + 4095) / 4096;
         if ((rc = dev_alloc(e, &e->key, n)) || (rc = dev_alloc(e, &e->key_sorted, n)) ||
             (rc = dev_alloc(e, &e->count, e->ncell1)) || (rc = dev_alloc(e, &e->start, e->ncell1)) ||
             (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->nchunks, 1)) ||
             (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) ||
-            (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)))
+            (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)) ||
+            (rc = dev_alloc(e, &e->flocks_dev, e->flocks.size())))
             return bail(rc);
+        if (e->flocks.size() > 1) {
+            if ((rc = dev_alloc(e, &e->flock_of, n))) return bail(rc);
+            std::vector<uint32_t> fo(n);
+            for (uint32_t k = 0; k < e->flocks.size(); ++k)
+                std::fill(fo.begin() + e->flocks[k].boid_base, fo.begin() + e->flocks[k].boid_base + e->flocks[k].P.n, k);
+            st = hipMemcpy(e->flock_of, fo.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+            if (st != hipSuccess) { hip_fail(e, st, "hipMemcpy"); return bail(BS_E_HIP); }
+        }
+        st = hipMemcpy(e->flocks_dev, e->flocks.data(), sizeof(FlockDesc) * e->flocks.size(), hipMemcpyHostToDevice);
+        if (st != hipSuccess) { hip_fail(e, st, "hipMemcpy"); return bail(BS_E_HIP); }
         if (e->mode == BS_MODE_GRID_MIXED) {
             if ((rc = dev_alloc(e, &e->lat.p, n)) || (rc = dev_alloc(e, &e->lat.vx, n)) ||
                 (rc = dev_alloc(e, &e->lat.vy, n)) || (rc = dev_alloc(e, &e->lat.vz, n)))
@@ -548,6 +627,50 @@ extern "C" int bs_create(const bs_params* p, bs_engine** out)
     return BS_OK;
 }
 
+extern "C" int bs_create(const bs_params* p, bs_engine** out)
+{
+    if (!out) return fail(nullptr, BS_E_ARG, "bs_create: out is NULL");
+    *out = nullptr;
+    if (!p) return fail(nullptr, BS_E_ARG, "bs_create: params is NULL");
+    return create_engine(p, 1, false, out);
+}
+
+extern "C" int bs_create_batch(const bs_params* flocks, uint32_t count, bs_engine** out)
+{
+    if (!out) return fail(nullptr, BS_E_ARG, "bs_create_batch: out is NULL");
+    *out = nullptr;
+    if (!flocks || count == 0) return fail(nullptr, BS_E_ARG, "bs_create_batch: no flocks");
+    return create_engine(flocks, count, true, out);
+}
+
+extern "C" int bs_get_flock_count(const bs_engine* e, uint32_t* count)
+{
+    if (!e || !count) return BS_E_ARG;
+    *count = e->flocks.empty() ? 1u : (uint32_t)e->flocks.size();
+    return BS_OK;
+}
+
+extern "C" int bs_get_flock_info(const bs_engine* e, uint32_t k, bs_flock_info* out)
+{
+    if (!e || !out) return BS_E_ARG;
+    memset(out, 0, sizeof(*out));
+    if (e->flocks.empty()) {
+        if (k != 0) return BS_E_ARG;
+        out->n = e->prm.n;
+        out->field = e->prm.field;
+        return BS_OK;
+    }
+    if (k >= e->flocks.size()) return BS_E_ARG;
+    const FlockDesc& d = e->flocks[k];
+    out->boid_base = d.boid_base;
+    out->n = d.P.n;
+    out->cells_per_axis = d.P.G;
+    out->cell_base = d.cell_base;
+    out->field = e->flock_field[k];
+    out->cell_edge = e->flock_field[k] / d.P.G;
+    return BS_OK;
+}
+
 extern "C" void bs_destroy(bs_engine* e) { release(e); }
 
 extern "C" const char* bs_last_error(const bs_engine* e)
@@ -565,13 +688,13 @@ static int grid_sort_into_a(bs_engine* e)
 {
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
-    HIP_TRY(e, launch_rows(e->P, e->start, e->table, e->nchunks, e->stream));
+    HIP_TRY(e, launch_rows(flock_view(e), e->start, e->table, e->nchunks, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
-    HIP_TRY(e, launch_scatter(e->P, e->key, e->start, e->count, e->db.id, e->perm, e->permid, e->stream));
+    HIP_TRY(e, launch_scatter(e->prm.n, e->key, e->start, e->count, e->db.id, e->perm, e->permid, e->stream));
     t2.stop();
     Timed t3(e, BS_K_GATHER);
-    HIP_TRY(e, launch_gather(e->P, e->perm, e->permid, e->key, e->start, e->count, e->db, e->da,
+    HIP_TRY(e, launch_gather(flock_view(e), e->perm, e->permid, e->key, e->start, e->count, e->db, e->da,
                              e->key_sorted, e->mode == BS_MODE_GRID_MIXED ? &e->lat : nullptr, e->stream));
     t3.stop();
     return BS_OK;
@@ -588,7 +711,7 @@ static int upload_f64(bs_engine* e, const double* pos3, const double* vel3)
         vx[i] = vel3[3 * (size_t)i]; vy[i] = vel3[3 * (size_t)i + 1]; vz[i] = vel3[3 * (size_t)i + 2];
         ids[i] = i;
     }
-    const bool grid = e->mode == BS_MODE_GRID_F64 || e->mode == BS_MODE_GRID_MIXED;
+    const bool grid = is_grid(e->mode);
     StateF64 dst = grid ? e->db : e->da;
     const size_t b = sizeof(double) * n;
     HIP_TRY(e, hipMemcpyAsync(dst.x, x, b, hipMemcpyHostToDevice, e->stream));
@@ -599,7 +722,7 @@ static int upload_f64(bs_engine* e, const double* pos3, const double* vel3)
     HIP_TRY(e, hipMemcpyAsync(dst.vz, vz, b, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(dst.id, ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, e->stream));
     if (grid) {
-        HIP_TRY(e, launch_hash_count(e->P, e->db, e->key, e->count, e->stream));
+        HIP_TRY(e, launch_hash_count(flock_view(e), e->db, e->key, e->count, e->stream));
         int rc = grid_sort_into_a(e);
         if (rc) return rc;
     }
@@ -738,10 +861,59 @@ extern "C" int bs_get_digest(bs_engine* e, uint64_t* d4)
 // ----------------------------------------------------------------------------
 // the step
 // ----------------------------------------------------------------------------
+// One grid update: fused neighbour pass (A -> B, with the next cell keys
+// and counts), then the counting sort of B back into A.  Every launch
+// reads and writes the same buffers each step, so the sequence can be
+// captured once and replayed.
+static int grid_step(bs_engine* e)
+{
+    Timed t(e, BS_K_PAIR);
+    const Flocks F = flock_view(e);
+    if (e->mode == BS_MODE_GRID_F64)
+        HIP_TRY(e, launch_nbr_f64(F, e->max_g, e->da, e->db, e->key_sorted, e->start, e->table, e->nchunks,
+                                  e->max_chunks, e->key, e->count, (double*)e->forces, e->digest, e->stream));
+    else
+        HIP_TRY(e, launch_nbr_mixed(F, e->max_g, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
+                                    e->nchunks, e->max_chunks, e->key, e->count, (double*)e->forces,
+                                    e->digest, e->recheck, e->stream));
+    t.stop();
+    return grid_sort_into_a(e);
+}
+
+// Captures grid_step into e->graph (once per engine).  The step holds no
+// memset node: the scan clears the counters the next step accumulates.
+static int capture_step(bs_engine* e)
+{
+    if (e->graph) return BS_OK;
+    HIP_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    e->capturing = true;
+    int rc = grid_step(e);
+    e->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t st = hipStreamEndCapture(e->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (st != hipSuccess) return hip_fail(e, st, "hipStreamEndCapture");
+    st = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (st != hipSuccess) { e->graph = nullptr; return hip_fail(e, st, "hipGraphInstantiate"); }
+    return BS_OK;
+}
+
 extern "C" int bs_step(bs_engine* e, uint32_t nsteps)
 {
     if (!e) return BS_E_ARG;
     if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    if (is_grid(e->mode) && (e->prm.flags & BS_FLAG_GRAPH) && nsteps) {
+        int rc = capture_step(e);
+        if (rc) return rc;
+        Timed t(e, BS_K_STEP);
+        for (uint32_t k = 0; k < nsteps; ++k) {
+            HIP_TRY(e, hipGraphLaunch(e->graph, e->stream));
+            e->steps++;
+        }
+        t.stop();
+        return BS_OK;
+    }
     for (uint32_t k = 0; k < nsteps; ++k) {
         switch (e->mode) {
         case BS_MODE_ALLPAIRS_F32REF: {
@@ -760,16 +932,7 @@ extern "C" int bs_step(bs_engine* e, uint32_t nsteps)
         }
         case BS_MODE_GRID_F64:
         case BS_MODE_GRID_MIXED: {
-            Timed t(e, BS_K_PAIR);
-            if (e->mode == BS_MODE_GRID_F64)
-                HIP_TRY(e, launch_nbr_f64(e->P, e->da, e->db, e->key_sorted, e->start, e->table, e->nchunks,
-                                          e->max_chunks, e->key, e->count, (double*)e->forces, e->digest, e->stream));
-            else
-                HIP_TRY(e, launch_nbr_mixed(e->P, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
-                                            e->nchunks, e->max_chunks, e->key, e->count, (double*)e->forces,
-                                            e->digest, e->recheck, e->stream));
-            t.stop();
-            int rc = grid_sort_into_a(e);
+            int rc = grid_step(e);
             if (rc) return rc;
             break;
         }
@@ -878,30 +1041,36 @@ extern "C" int bs_debug_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
 }
 
 // Flock metrics of the current state (see boids_metrics.hip).  Temporary
-// device memory is allocated and released inside the call.
-extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out)
+// device memory is allocated and released inside the call.  Flock k of a
+// batch is the id range [boid_base, boid_base + n) of the unsorted state.
+extern "C" int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out)
 {
     if (!e || !out) return BS_E_ARG;
     if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
     if (!(eps >= 0.0) || !isfinite(eps)) return fail(e, BS_E_ARG, "eps must be finite and >= 0");
-    const uint32_t n = e->prm.n;
+    const uint32_t nflocks = e->flocks.empty() ? 1u : (uint32_t)e->flocks.size();
+    if (flock >= nflocks) return fail(e, BS_E_ARG, "flock index out of range");
+    const uint32_t ntot = e->prm.n;
+    const uint32_t base = e->flocks.empty() ? 0u : e->flocks[flock].boid_base;
+    const uint32_t n = e->flocks.empty() ? ntot : e->flocks[flock].P.n;
+    const double F = e->flocks.empty() ? e->prm.field : e->flock_field[flock];
     memset(out, 0, sizeof(*out));
     if (n == 0) return BS_OK;
-    const double F = e->prm.field;
     // edge >= eps (links need only the 27 neighbouring cells); no more cells than ~8 per boid
     double g = eps > 0.0 ? floor(F / (eps * 1.0001)) : 1024.0;
     const double cap = std::max(1.0, 2.0 * floor(cbrt((double)n)));
     g = std::min(std::min(g, cap), 1024.0);
     const int G = std::max(1, (int)g);
     if (e->mode == BS_MODE_ALLPAIRS_F32REF)
-        HIP_TRY(e, launch_f32_to_f64(n, e->fa, e->tmp_pos, e->tmp_vel, e->stream));
+        HIP_TRY(e, launch_f32_to_f64(ntot, e->fa, e->tmp_pos, e->tmp_vel, e->stream));
     else
-        HIP_TRY(e, launch_unsort_f64(n, e->da, e->tmp_pos, e->tmp_vel, e->stream));
+        HIP_TRY(e, launch_unsort_f64(ntot, e->da, e->tmp_pos, e->tmp_vel, e->stream));
     void* ws = nullptr;
     const size_t bytes = metrics_workspace_bytes(n, G);
     if (hipMalloc(&ws, bytes) != hipSuccess) return fail(e, BS_E_NOMEM, "metrics workspace allocation failed");
     double* res = nullptr;
-    hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos, e->tmp_vel, ws, bytes, &res, e->stream);
+    hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos + 3 * (size_t)base,
+                                   e->tmp_vel + 3 * (size_t)base, ws, bytes, &res, e->stream);
     double host[4] = {0, 0, 0, 0};
     if (st == hipSuccess) st = hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, e->stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
@@ -912,4 +1081,11 @@ extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metric
     out->clusters = (uint64_t)host[2];
     out->largest = (uint64_t)host[3];
     return BS_OK;
+}
+
+extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out)
+{
+    if (!e || !out) return BS_E_ARG;
+    if (e->flocks.size() > 1) return fail(e, BS_E_STATE, "batched engine: use bs_metrics_flock");
+    return bs_metrics_flock(e, 0, eps, min_size, out);
 }

@@ -80,29 +80,45 @@ struct Lattice {               // mixed mode companion of the sorted FP64 state
     float *vx, *vy, *vz;       // FP32 copies of the velocity
 };
 
+// One flock of a grid engine.  A grid engine simulates one or more
+// independent flocks (a parameter sweep); flock f owns the boid ids/slots
+// [boid_base, boid_base + P.n) and the global cell keys
+// [cell_base, cell_base + G^3), so one counting sort over global keys keeps
+// every flock contiguous.  (y,z) rows are numbered globally from row_base.
+struct FlockDesc {
+    DevParams P;
+    uint32_t boid_base, cell_base, row_base, nrows;
+};
+
+// Kernel view of the flock table.
+struct Flocks {
+    const FlockDesc* fl;       // [count] in device memory
+    const uint32_t* flock_of;  // flock of each slot (nullptr: one flock)
+    uint32_t count;            // number of flocks
+    uint32_t n;                // boids over all flocks
+    uint32_t rows;             // (y,z) rows over all flocks
+};
+
 // Launchers (boids_kernels.hip).  All launch on `stream`.
 hipError_t launch_allpairs_f32ref(const DevParams& P, StateF32 in, StateF32 out, float* forces,
                                   uint64_t* digest, hipStream_t stream);
 hipError_t launch_allpairs_f64(const DevParams& P, StateF64 in, StateF64 out, double* forces,
                                uint64_t* digest, hipStream_t stream);
-hipError_t launch_hash_count(const DevParams& P, StateF64 st, uint32_t* key, uint32_t* count,
-                             hipStream_t stream);
+hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* count, hipStream_t stream);
 hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
                        hipStream_t stream);
-hipError_t launch_rows(const DevParams& P, const uint32_t* start, uint4* table, uint32_t* nchunks,
+hipError_t launch_rows(Flocks F, const uint32_t* start, uint4* table, uint32_t* nchunks,
                        hipStream_t stream);
-hipError_t launch_scatter(const DevParams& P, const uint32_t* key, const uint32_t* start,
-                          uint32_t* count, const uint32_t* id, uint32_t* perm, uint32_t* permid,
-                          hipStream_t stream);
-hipError_t launch_gather(const DevParams& P, const uint32_t* perm, const uint32_t* permid,
-                         const uint32_t* key, const uint32_t* start, uint32_t* count,
-                         StateF64 src, StateF64 dst, uint32_t* key_sorted, Lattice* lat,
-                         hipStream_t stream);
-hipError_t launch_nbr_f64(const DevParams& P, StateF64 in, StateF64 out, const uint32_t* key_sorted,
+hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
+                          const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
+hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,
+                         const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
+                         uint32_t* key_sorted, Lattice* lat, hipStream_t stream);
+hipError_t launch_nbr_f64(Flocks F, int max_g, StateF64 in, StateF64 out, const uint32_t* key_sorted,
                           const uint32_t* start, const uint4* table, const uint32_t* nchunks,
                           uint32_t max_chunks, uint32_t* key_out, uint32_t* count, double* forces,
                           uint64_t* digest, hipStream_t stream);
-hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Lattice lat,
+hipError_t launch_nbr_mixed(Flocks F, int max_g, StateF64 in, StateF64 out, Lattice lat,
                             const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                             const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,

@@ -214,14 +214,21 @@ __global__ __launch_bounds__(kBlock) void k_allpairs(DevParams P, St in, St out,
 
 // ---------------------------------------------------------------------------
 // Grid build: cell hash + count, exclusive scan, scatter, in-cell rank +
-// gather.  Sorted order is (cell, boid id), independent of history.
+// gather.  Sorted order is (cell, boid id), independent of history.  Keys
+// are global over the flock table (flock cell_base + local cell).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_hash_count(DevParams P, StateF64 st, uint32_t* key,
+__device__ __forceinline__ uint32_t flock_of(const Flocks& F, uint32_t slot)
+{
+    return F.flock_of ? F.flock_of[slot] : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hash_count(Flocks F, StateF64 st, uint32_t* key,
                                                         uint32_t* count)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    const uint32_t c = cell_of(P, st.x[i], st.y[i], st.z[i]);
+    if (i >= F.n) return;
+    const FlockDesc& fd = F.fl[flock_of(F, i)];
+    const uint32_t c = fd.cell_base + cell_of(fd.P, st.x[i], st.y[i], st.z[i]);
     key[i] = c;
     atomicAdd(&count[c], 1u);
 }
@@ -282,20 +289,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
     }
 }
 
-// Block table of the neighbour kernels: every (y,z) row of cells is split
-// into ceil(n_row/kBlock) chunks of consecutive sorted boids of near-equal
-// size, so that no block spans two rows.  One workgroup.
-__global__ __launch_bounds__(kScanThreads) void k_rows(DevParams P, const uint32_t* start,
+// Block table of the neighbour kernels: every (y,z) row of cells of every
+// flock is split into ceil(n_row/kNbr) chunks of consecutive sorted boids
+// of near-equal size, so that no block spans two rows.  Entries are
+// (local row, first slot, end slot, flock).  One workgroup.
+__global__ __launch_bounds__(kScanThreads) void k_rows(Flocks F, const uint32_t* start,
                                                         uint4* table, uint32_t* nchunks)
 {
     __shared__ uint32_t wsum[17];
-    const uint32_t G = (uint32_t)P.G;
-    const uint32_t rows = G * G;
     uint32_t carry = 0;
-    for (uint32_t b = 0; b < rows; b += kScanThreads) {
+    for (uint32_t b = 0; b < F.rows; b += kScanThreads) {
         const uint32_t r = b + threadIdx.x;
-        uint32_t lo = 0, nr = 0;
-        if (r < rows) { lo = start[r * G]; nr = start[(r + 1) * G] - lo; }
+        uint32_t lo = 0, nr = 0, f = 0, rl = 0;
+        if (r < F.rows) {
+            uint32_t a = 0, z = F.count - 1;              // last flock with row_base <= r
+            while (a < z) {
+                const uint32_t m = (a + z + 1) >> 1;
+                if (F.fl[m].row_base <= r) a = m; else z = m - 1;
+            }
+            f = a;
+            const FlockDesc& fd = F.fl[f];
+            const uint32_t G = (uint32_t)fd.P.G;
+            rl = r - fd.row_base;
+            const uint32_t c0 = fd.cell_base + rl * G;
+            lo = start[c0];
+            nr = start[c0 + G] - lo;
+        }
         const uint32_t kr = (nr + kNbr - 1) / kNbr;
         uint32_t total;
         const uint32_t off = block_exclusive_scan(kr, wsum, &total) + carry;
@@ -304,7 +323,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rows(DevParams P, const uint32
             for (uint32_t k = 0; k < kr; ++k) {
                 const uint32_t t0 = lo + k * csz;
                 const uint32_t t1 = min(lo + nr, t0 + csz);
-                table[off + k] = make_uint4(r, t0, t1, 0u);
+                table[off + k] = make_uint4(rl, t0, t1, f);
             }
         }
         carry += total;
@@ -332,14 +351,14 @@ __device__ __forceinline__ uint32_t to_lattice(double x, double scale)
 
 // Final slot = cell start + rank of the boid id among its cell's ids, so
 // the order inside a cell is ascending id whatever the atomics did.
-__global__ __launch_bounds__(kBlock) void k_gather(DevParams P, const uint32_t* perm,
+__global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* perm,
                                                    const uint32_t* permid, const uint32_t* key,
                                                    const uint32_t* start, uint32_t* count,
                                                    StateF64 src, StateF64 dst,
                                                    uint32_t* key_sorted, Lattice lat, int with_lat)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= P.n) return;
+    if (t >= F.n) return;
     const uint32_t s = perm[t];
     const uint32_t me = permid[t];
     const uint32_t c = key[s];
@@ -355,8 +374,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevParams P, const uint32_t* 
     key_sorted[f] = c;
     if (rank == 0) count[c] = 0u;
     if (with_lat) {
-        lat.p[f] = make_uint4(to_lattice(x, P.lat_scale), to_lattice(y, P.lat_scale),
-                              to_lattice(z, P.lat_scale), f);
+        const double scale = F.fl[flock_of(F, f)].P.lat_scale;   // slots stay inside their flock
+        lat.p[f] = make_uint4(to_lattice(x, scale), to_lattice(y, scale), to_lattice(z, scale), f);
         lat.vx[f] = (float)vx; lat.vy[f] = (float)vy; lat.vz[f] = (float)vz;
     }
 }
@@ -402,9 +421,10 @@ __device__ __forceinline__ Segment segment_view(unsigned char* dyn, int G, uint3
     return S;
 }
 
-// Builds the segment of neighbour row `nrow` for cells cxa-1 .. cxa+nseg-2.
-__device__ __forceinline__ void build_segment(Segment& S, const uint32_t* start, int G, int cxa,
-                                              int nseg, uint32_t nrow)
+// Builds the segment of neighbour row `nrow` (of the flock whose cells start
+// at global key cbase) for cells cxa-1 .. cxa+nseg-2.
+__device__ __forceinline__ void build_segment(Segment& S, const uint32_t* start, uint32_t cbase, int G,
+                                              int cxa, int nseg, uint32_t nrow)
 {
     // counts, then a block-wide exclusive scan (each thread owns up to
     // ceil(nseg/kNbr) consecutive cells)
@@ -415,7 +435,7 @@ __device__ __forceinline__ void build_segment(Segment& S, const uint32_t* start,
         int cx = cxa - 1 + k;
         int sh = 0;
         if (cx < 0) { cx += G; sh = -1; } else if (cx >= G) { cx -= G; sh = 1; }
-        const uint32_t c = (uint32_t)cx + (uint32_t)G * nrow;
+        const uint32_t c = cbase + (uint32_t)cx + (uint32_t)G * nrow;
         const uint32_t a = start[c], b = start[c + 1];
         S.segg[k] = a;
         S.segsh[k] = (int8_t)sh;
@@ -445,9 +465,9 @@ __device__ __forceinline__ int seg_cell(const Segment& S, int nseg, uint32_t g)
 }
 
 // Phase-B + outputs shared by the FP64 and mixed neighbour kernels.
-__device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t s, uint32_t id, double px,
-                                            double py, double pz, double vx, double vy, double vz,
-                                            double cx_, double cy_, double cz_, double sx_,
+__device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t cbase, uint32_t s, uint32_t id,
+                                            double px, double py, double pz, double vx, double vy,
+                                            double vz, double cx_, double cy_, double cz_, double sx_,
                                             double sy_, double sz_, double ax_, double ay_,
                                             double az_, StateF64 out, uint32_t* key_out,
                                             uint32_t* count, double* forces)
@@ -463,7 +483,7 @@ __device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t s, uint
     out.x[s] = px; out.y[s] = py; out.z[s] = pz;
     out.vx[s] = vx; out.vy[s] = vy; out.vz[s] = vz;
     out.id[s] = id;
-    const uint32_t c = cell_of(P, px, py, pz);
+    const uint32_t c = cbase + cell_of(P, px, py, pz);
     key_out[s] = c;
     atomicAdd(&count[c], 1u);
 }
@@ -480,7 +500,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t grid)
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, StateF64 in, StateF64 out,
+__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, StateF64 in, StateF64 out,
                                                   const uint32_t* __restrict__ key_sorted,
                                                   const uint32_t* __restrict__ start,
                                                   const uint4* __restrict__ table,
@@ -497,14 +517,17 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, 
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= *nchunks) return;
     const uint4 ch = table[b];
+    const FlockDesc& fd = FL.fl[ch.w];
+    const DevParams& P = fd.P;
+    const uint32_t cbase = fd.cell_base;
     const int G = P.G;
     Segment S = segment_view(dyn, G, wsum);
     const uint32_t row = ch.x, t0 = ch.y, t1 = ch.z;
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
     const uint32_t s = t0 + threadIdx.x;
     const bool active = s < t1;
-    const int cxa = (int)(key_sorted[t0] % (uint32_t)G);
-    const int cxb = (int)(key_sorted[t1 - 1] % (uint32_t)G);
+    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
     const int nseg = cxb - cxa + 3;
 
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
@@ -516,7 +539,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, 
         myid = in.id[s];
         mykey = key_sorted[s];
         mystart = start[mykey];
-        L = (int)(mykey % (uint32_t)G) - cxa + 1;
+        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
     }
     const double vn = norm3(vx, vy, vz);
     const double F = P.F;
@@ -535,7 +558,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, 
         if (nz < 0) { nz += G; shz = -F; } else if (nz >= G) { nz -= G; shz = F; }
         const uint32_t nrow = (uint32_t)ny + (uint32_t)G * (uint32_t)nz;
         __syncthreads();                      // previous readers of S and the tile are done
-        build_segment(S, start, G, cxa, nseg, nrow);
+        build_segment(S, start, cbase, G, cxa, nseg, nrow);
         const uint32_t total = S.segc[nseg];
         const uint32_t wlo = S.segc[L - 1], whi = S.segc[L + 2];
         const uint32_t selfk = (rr == 4) ? S.segc[L] + (s - mystart) : 0xFFFFFFFFu;
@@ -641,7 +664,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_f64(DevParams P, 
         d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
         d[1] = hc; d[2] = hs; d[3] = ha;
     }
-    finish_boid(P, s, myid, px, py, pz, vx, vy, vz, scx, scy, scz, ssx, ssy, ssz, sax, say, saz,
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, scx, scy, scz, ssx, ssy, ssz, sax, say, saz,
                 out, key_out, count, forces);
 }
 
@@ -699,7 +722,7 @@ __device__ __forceinline__ int rule_verdict(const DevParams& P, int r, float s, 
 }
 
 template <bool DIGEST>
-__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P, StateF64 in, StateF64 out,
+__global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, StateF64 in, StateF64 out,
                                                     Lattice lat,
                                                     const uint32_t* __restrict__ key_sorted,
                                                     const uint32_t* __restrict__ start,
@@ -720,14 +743,17 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= *nchunks) return;
     const uint4 ch = table[b];
+    const FlockDesc& fd = FL.fl[ch.w];
+    const DevParams& P = fd.P;
+    const uint32_t cbase = fd.cell_base;
     const int G = P.G;
     Segment S = segment_view(dyn, G, wsum);
     const uint32_t row = ch.x, t0 = ch.y, t1 = ch.z;
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
     const uint32_t s = t0 + threadIdx.x;
     const bool active = s < t1;
-    const int cxa = (int)(key_sorted[t0] % (uint32_t)G);
-    const int cxb = (int)(key_sorted[t1 - 1] % (uint32_t)G);
+    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
     const int nseg = cxb - cxa + 3;
 
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
@@ -743,7 +769,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P
         fvx = lat.vx[s]; fvy = lat.vy[s]; fvz = lat.vz[s];
         const uint32_t mykey = key_sorted[s];
         mystart = start[mykey];
-        L = (int)(mykey % (uint32_t)G) - cxa + 1;
+        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
     }
     const v2s my_xy = __builtin_bit_cast(v2s, (me.x >> 16) | (me.y & 0xFFFF0000u));
     const v2s my_z = __builtin_bit_cast(v2s, me.z >> 16);
@@ -768,7 +794,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P
         if (nz < 0) nz += G; else if (nz >= G) nz -= G;
         const uint32_t nrow = (uint32_t)ny + (uint32_t)G * (uint32_t)nz;
         __syncthreads();
-        build_segment(S, start, G, cxa, nseg, nrow);
+        build_segment(S, start, cbase, G, cxa, nseg, nrow);
         const uint32_t total = S.segc[nseg];
         const uint32_t wlo = S.segc[L - 1], whi = S.segc[L + 2];
         const uint32_t selfk = (rr == 4) ? S.segc[L] + (s - mystart) : 0xFFFFFFFFu;
@@ -884,7 +910,7 @@ __global__ __launch_bounds__(kNbr) BS_NBR_OCCUPANCY void k_nbr_mixed(DevParams P
         d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
         d[1] = hc; d[2] = hs; d[3] = ha;
     }
-    finish_boid(P, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, dsx, dsy, dsz, dax, day, daz,
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, dsx, dsy, dsz, dax, day, daz,
                 out, key_out, count, forces);
 }
 
@@ -911,11 +937,10 @@ hipError_t launch_allpairs_f64(const DevParams& P, StateF64 in, StateF64 out, do
     return hipGetLastError();
 }
 
-hipError_t launch_hash_count(const DevParams& P, StateF64 st, uint32_t* key, uint32_t* count,
-                             hipStream_t stream)
+hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* count, hipStream_t stream)
 {
-    if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hash_count, dim3(blocks_for(P.n)), dim3(kBlock), 0, stream, P, st, key, count);
+    if (F.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash_count, dim3(blocks_for(F.n)), dim3(kBlock), 0, stream, F, st, key, count);
     return hipGetLastError();
 }
 
@@ -930,68 +955,69 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
     return hipGetLastError();
 }
 
-hipError_t launch_rows(const DevParams& P, const uint32_t* start, uint4* table, uint32_t* nchunks,
+hipError_t launch_rows(Flocks F, const uint32_t* start, uint4* table, uint32_t* nchunks,
                        hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_rows, dim3(1), dim3(kScanThreads), 0, stream, P, start, table, nchunks);
+    if (F.count == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rows, dim3(1), dim3(kScanThreads), 0, stream, F, start, table, nchunks);
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const DevParams& P, const uint32_t* key, const uint32_t* start,
-                          uint32_t* count, const uint32_t* id, uint32_t* perm, uint32_t* permid,
-                          hipStream_t stream)
+hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
+                          const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream)
 {
-    if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n)), dim3(kBlock), 0, stream, P.n, key, start,
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, stream, n, key, start,
                        count, id, perm, permid);
     return hipGetLastError();
 }
 
-hipError_t launch_gather(const DevParams& P, const uint32_t* perm, const uint32_t* permid,
-                         const uint32_t* key, const uint32_t* start, uint32_t* count,
-                         StateF64 src, StateF64 dst, uint32_t* key_sorted, Lattice* lat,
-                         hipStream_t stream)
+hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,
+                         const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
+                         uint32_t* key_sorted, Lattice* lat, hipStream_t stream)
 {
-    if (P.n == 0) return hipSuccess;
+    if (F.n == 0) return hipSuccess;
     Lattice l = lat ? *lat : Lattice{nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(P.n)), dim3(kBlock), 0, stream, P, perm, permid,
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(F.n)), dim3(kBlock), 0, stream, F, perm, permid,
                        key, start, count, src, dst, key_sorted, l, lat ? 1 : 0);
     return hipGetLastError();
 }
 
 static inline uint32_t round8(uint32_t g) { return (g + 7u) & ~7u; }
 
-hipError_t launch_nbr_f64(const DevParams& P, StateF64 in, StateF64 out, const uint32_t* key_sorted,
+hipError_t launch_nbr_f64(Flocks F, int max_g, StateF64 in, StateF64 out, const uint32_t* key_sorted,
                           const uint32_t* start, const uint4* table, const uint32_t* nchunks,
                           uint32_t max_chunks, uint32_t* key_out, uint32_t* count, double* forces,
                           uint64_t* digest, hipStream_t stream)
 {
-    if (P.n == 0) return hipSuccess;
+    if (F.n == 0) return hipSuccess;
+    if (max_g < 3 || max_g > kMaxCells) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = segment_bytes(P.G);
+    const size_t dyn = segment_bytes(max_g);
     if (digest)
-        hipLaunchKernelGGL(k_nbr_f64<true>, grid, dim3(kNbr), dyn, stream, P, in, out, key_sorted,
+        hipLaunchKernelGGL(k_nbr_f64<true>, grid, dim3(kNbr), dyn, stream, F, in, out, key_sorted,
                            start, table, nchunks, key_out, count, forces, digest);
     else
-        hipLaunchKernelGGL(k_nbr_f64<false>, grid, dim3(kNbr), dyn, stream, P, in, out, key_sorted,
+        hipLaunchKernelGGL(k_nbr_f64<false>, grid, dim3(kNbr), dyn, stream, F, in, out, key_sorted,
                            start, table, nchunks, key_out, count, forces, digest);
     return hipGetLastError();
 }
 
-hipError_t launch_nbr_mixed(const DevParams& P, StateF64 in, StateF64 out, Lattice lat,
+hipError_t launch_nbr_mixed(Flocks F, int max_g, StateF64 in, StateF64 out, Lattice lat,
                             const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                             const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
                             hipStream_t stream)
 {
-    if (P.n == 0) return hipSuccess;
+    if (F.n == 0) return hipSuccess;
+    if (max_g < 3 || max_g > kMaxCells) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = segment_bytes(P.G);
+    const size_t dyn = segment_bytes(max_g);
     if (digest)
-        hipLaunchKernelGGL(k_nbr_mixed<true>, grid, dim3(kNbr), dyn, stream, P, in, out, lat,
+        hipLaunchKernelGGL(k_nbr_mixed<true>, grid, dim3(kNbr), dyn, stream, F, in, out, lat,
                            key_sorted, start, table, nchunks, key_out, count, forces, digest, recheck);
     else
-        hipLaunchKernelGGL(k_nbr_mixed<false>, grid, dim3(kNbr), dyn, stream, P, in, out, lat,
+        hipLaunchKernelGGL(k_nbr_mixed<false>, grid, dim3(kNbr), dyn, stream, F, in, out, lat,
                            key_sorted, start, table, nchunks, key_out, count, forces, digest, recheck);
     return hipGetLastError();
 }

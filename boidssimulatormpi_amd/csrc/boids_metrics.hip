@@ -302,9 +302,7 @@ hipError_t launch_metrics(uint32_t n, double F, int G, double eps, uint32_t min_
     hipError_t st = launch_scan(count, start, cells, partials, stream);
     if (st != hipSuccess) return st;
     hipLaunchKernelGGL(k_m_iota, dim3(mblocks(n)), dim3(kBlock), 0, stream, ids, n);
-    DevParams P{};
-    P.n = n;
-    st = launch_scatter(P, key, start, count, ids, perm, permid, stream);
+    st = launch_scatter(n, key, start, count, ids, perm, permid, stream);
     if (st != hipSuccess) return st;
     hipLaunchKernelGGL(k_m_order, dim3(mblocks(n)), dim3(kBlock), 0, stream, g, pos3, (const uint32_t*)permid, cpos);
     hipLaunchKernelGGL(k_m_nn, dim3(mblocks(n)), dim3(kBlock), 0, stream, g, pos3, (const uint32_t*)key,

@@ -34,9 +34,10 @@ INIT_CLUSTERED = 3
 FLAG_FORCES = 1
 FLAG_DIGEST = 2
 FLAG_TIMING = 4
+FLAG_GRAPH = 8
 
-K_PAIR, K_SCAN, K_SCATTER, K_GATHER, K_OTHER = range(5)
-K_NAMES = ["pair", "scan", "scatter", "gather", "other"]
+K_PAIR, K_SCAN, K_SCATTER, K_GATHER, K_OTHER, K_STEP = range(6)
+K_NAMES = ["pair", "scan", "scatter", "gather", "other", "step"]
 
 
 class Params(ctypes.Structure):
@@ -55,6 +56,12 @@ class Info(ctypes.Structure):
                 ("steps", ctypes.c_uint64)]
 
 
+class FlockInfo(ctypes.Structure):
+    _fields_ = [("boid_base", ctypes.c_uint32), ("n", ctypes.c_uint32),
+                ("cells_per_axis", ctypes.c_int32), ("cell_base", ctypes.c_uint32),
+                ("field", ctypes.c_double), ("cell_edge", ctypes.c_double)]
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -71,7 +78,8 @@ EXPORTS = ["bs_create", "bs_destroy", "bs_last_error", "bs_generate_init", "bs_r
            "bs_set_state_f32", "bs_set_state_f64", "bs_step", "bs_sync", "bs_get_state_f32",
            "bs_get_state_f64", "bs_get_forces_f32", "bs_get_forces_f64", "bs_get_digest",
            "bs_get_info", "bs_get_timing", "bs_reset_timing", "bs_debug_thresholds",
-           "bs_debug_rechecks", "bs_debug_copy", "bs_metrics", "bs_version"]
+           "bs_debug_rechecks", "bs_debug_copy", "bs_metrics", "bs_version", "bs_create_batch",
+           "bs_get_flock_count", "bs_get_flock_info", "bs_metrics_flock"]
 
 
 class Metrics(ctypes.Structure):
@@ -116,6 +124,10 @@ def lib():
         "bs_debug_copy": (c_int, [ctypes.c_uint64, ctypes.c_int32, f64p]),
         "bs_metrics": (c_int, [vp, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
         "bs_version": (ctypes.c_char_p, []),
+        "bs_create_batch": (c_int, [P(Params), ctypes.c_uint32, P(vp)]),
+        "bs_get_flock_count": (c_int, [vp, P(ctypes.c_uint32)]),
+        "bs_get_flock_info": (c_int, [vp, ctypes.c_uint32, P(FlockInfo)]),
+        "bs_metrics_flock": (c_int, [vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -165,13 +177,23 @@ def debug_copy(nbytes: int, reps: int = 1) -> float:
 class Engine:
     """Owns one bs_engine handle (device buffers + one HIP stream)."""
 
-    def __init__(self, params: Params):
-        self.params = params
-        self.n = params.n
+    def __init__(self, params):
+        """params: one Params (bs_create) or a sequence of Params, one per
+        flock of a parameter sweep (bs_create_batch)."""
         h = ctypes.c_void_p()
-        rc = lib().bs_create(ctypes.byref(params), ctypes.byref(h))
+        if isinstance(params, Params):
+            self.params = params
+            self.n = params.n
+            rc = lib().bs_create(ctypes.byref(params), ctypes.byref(h))
+            what = "bs_create"
+        else:
+            arr = (Params * len(params))(*params)
+            self.params = list(params)
+            self.n = sum(p.n for p in params)
+            rc = lib().bs_create_batch(arr, len(params), ctypes.byref(h))
+            what = "bs_create_batch"
         if rc != 0:
-            raise EngineError(f"bs_create failed ({rc}): {lib().bs_last_error(None).decode()}")
+            raise EngineError(f"{what} failed ({rc}): {lib().bs_last_error(None).decode()}")
         self._h = h
 
     def _check(self, rc, what):
@@ -256,12 +278,27 @@ class Engine:
         self._check(lib().bs_debug_rechecks(self._h, ctypes.byref(out)), "bs_debug_rechecks")
         return out.value
 
-    def metrics(self, eps, min_size=2) -> dict:
+    def metrics(self, eps, min_size=2, flock=None) -> dict:
         """Polarisation, mean nearest-neighbour distance, cluster count (graph
-        d < eps, components >= min_size) and largest cluster of the state."""
+        d < eps, components >= min_size) and largest cluster of the state
+        (of flock `flock` of a batch)."""
         m = Metrics()
-        self._check(lib().bs_metrics(self._h, eps, min_size, ctypes.byref(m)), "bs_metrics")
+        if flock is None:
+            self._check(lib().bs_metrics(self._h, eps, min_size, ctypes.byref(m)), "bs_metrics")
+        else:
+            self._check(lib().bs_metrics_flock(self._h, flock, eps, min_size, ctypes.byref(m)),
+                        "bs_metrics_flock")
         return m.as_dict()
+
+    def flock_count(self) -> int:
+        c = ctypes.c_uint32()
+        self._check(lib().bs_get_flock_count(self._h, ctypes.byref(c)), "bs_get_flock_count")
+        return c.value
+
+    def flock_info(self, k) -> FlockInfo:
+        i = FlockInfo()
+        self._check(lib().bs_get_flock_info(self._h, k, ctypes.byref(i)), "bs_get_flock_info")
+        return i
 
 
 class BoidSimulation:

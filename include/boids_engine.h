@@ -64,7 +64,10 @@ enum {
 enum {
     BS_FLAG_FORCES = 1,   /* keep per-boid dv_coh/dv_sep/dv_ali for get_forces (get_force) */
     BS_FLAG_DIGEST = 2,   /* keep per-boid neighbour digests (parity checks) */
-    BS_FLAG_TIMING = 4    /* record HIP events around every kernel launch */
+    BS_FLAG_TIMING = 4,   /* record HIP events around every kernel launch */
+    BS_FLAG_GRAPH = 8     /* grid modes: capture one update as a HIP graph on the
+                           * first bs_step and replay it (timing then covers
+                           * whole updates only, as BS_K_STEP) */
 };
 
 /* Parameters of BoidSimulation::setup (src/boid_simulation.h:19-33), same
@@ -101,12 +104,32 @@ enum {
     BS_K_SCATTER = 2,    /* counting-sort scatter */
     BS_K_GATHER = 3,     /* in-cell rank + state gather */
     BS_K_OTHER = 4,
-    BS_K_COUNT = 5
+    BS_K_STEP = 5,       /* whole graph-replayed updates (BS_FLAG_GRAPH) */
+    BS_K_COUNT = 6
 };
 
 /* ctor + setup() (src/boid_simulation.h:17-33).  *out is NULL on failure;
  * the message is then available from bs_last_error(NULL). */
 int bs_create(const bs_params* p, bs_engine** out);
+/* Parameter sweep: `count` independent flocks in one engine.  The
+ * reference runs a sweep as one process per parameter set
+ * (settings/parameter_search_vel_2/gen.py:32-42 writes the .ini files,
+ * tools/gen_jobscript.py:37-65 one job per file); here every flock keeps its
+ * own n, field, rules and velocity limits, and one launch per kernel steps
+ * them all.  Grid modes only; mode, flags and device must agree; every
+ * flock needs >= 3 cells per axis.  Boids are numbered flock after flock:
+ * flock k owns ids [boid_base, boid_base + n) of every state, force and
+ * digest array (bs_get_flock_info). */
+int bs_create_batch(const bs_params* flocks, uint32_t count, bs_engine** out);
+typedef struct {
+    uint32_t boid_base, n;
+    int32_t cells_per_axis;
+    uint32_t cell_base;          /* first global cell key of the flock */
+    double field, cell_edge;
+} bs_flock_info;
+int bs_get_flock_count(const bs_engine* e, uint32_t* count);
+int bs_get_flock_info(const bs_engine* e, uint32_t k, bs_flock_info* out);
+
 /* ~BoidSimulation() */
 void bs_destroy(bs_engine* e);
 /* Message of the last failed call on e (e == NULL: last bs_create failure). */
@@ -176,6 +199,8 @@ typedef struct {
     uint64_t clusters, largest;
 } bs_metrics_out;
 int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out);
+/* The same for flock k of a batch (bs_metrics fails on a batch of more than one flock). */
+int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out);
 
 /* Measurement helper: `reps` launches of a plain streaming copy of `bytes`
  * bytes on the current device (reference point for HBM byte counters).
