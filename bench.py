@@ -13,6 +13,12 @@ with the state resident in HBM; the FP64 plain figure is reported beside it.
 `roofline` is for the dominant kernel (fused neighbour + integrate) of the
 headline path, from HIP events on the engine's stream; `cpu_baseline` times
 the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
+
+`sweep` (with --sweep-steps > 0) reports the many-flock parameter sweep: 512
+flocks, N log-uniform in [2^10, 2^16], field (N/16384)^(1/3), vmax ~
+U[0.002, 0.01], vmin = vmax/5, radii x U[0.7, 1.3], mixed precision, seed 3,
+run as one batched engine with graph-replayed steps, next to the same
+kernels launched flock by flock (one engine per flock, eager launches).
 """
 from __future__ import annotations
 
@@ -40,8 +46,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(mode, init, n, field, steps, warmup):
-    e = B.Engine(B.make_params(n, field, **RULES, mode=mode, flags=E.FLAG_TIMING))
+def tuned_params(mode, init, n, field, tune, block=0):
+    """Engine parameters of one mode: the autotuner's choice (timed on this
+    workload's own initial state) or the defaults."""
+    p = B.make_params(n, field, **RULES, mode=mode, flags=E.FLAG_TIMING, block=block)
+    if not tune:
+        return p, None
+    from boidssimulatormpi_amd import autotune as T
+    best, ps, timings = T.tune(p, {"xyzv6": init}, steps=6, warmup=2, cache=False,
+                               blocks=(block,) if block else T.BLOCKS)
+    return ps[0], {"best": best, "candidates": len(timings),
+                   "best_ms": min(t["ms_per_step"] for t in timings),
+                   "worst_ms": max(t["ms_per_step"] for t in timings)}
+
+
+def run_mode(mode, init, n, field, steps, warmup, params):
+    e = B.Engine(params)
     info = e.info()
     e.set_state(xyzv6=init)
     e.step(warmup)
@@ -63,9 +83,63 @@ def run_mode(mode, init, n, field, steps, warmup):
     rechecks = e.rechecks()
     e.close()
     wall = t1 - t0
-    return {"mode": E.MODE_NAMES[mode], "cells_per_axis": info.cells_per_axis,
+    return {"mode": E.MODE_NAMES[mode], "cells_per_axis": info.cells_per_axis, "block": info.block,
+            "tile": info.tile, "fill": info.fill,
             "ms_per_step": 1e3 * wall / steps, "boid_steps_per_s": n * steps / wall,
             "kernels": kernels, "sane": ok, "fp64_rechecks": rechecks}
+
+
+def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
+    """Batched flock table vs one engine per flock on the synthetic sweep."""
+    from boidssimulatormpi_amd import sweep as S
+    jobs = S.generated_sweep(count, seed)
+    states = [S.initial_state(j) for j in jobs]
+    total = sum(j.n for j in jobs)
+    out = {"flocks": count, "boids": total, "seed": seed, "mode": "grid_mixed"}
+
+    sw = S.Sweep(jobs, mode=E.MODE_GRID_MIXED, flags=E.FLAG_GRAPH | E.FLAG_TIMING, block=block)
+    sw.set_initial_state(states)
+    sw.step(warmup)
+    sw.engine.sync()
+    sw.engine.reset_timing()
+    t0 = time.perf_counter()
+    sw.step(steps)
+    sw.engine.sync()
+    wall = time.perf_counter() - t0
+    ms_graph, _ = sw.engine.timing(E.K_STEP)
+    p, v = sw.engine.state_f64()
+    sane = bool(np.isfinite(p).all() and np.isfinite(v).all())
+    sw.close()
+    out["batched"] = {"steps": steps, "ms_per_step": 1e3 * wall / steps, "boid_steps_per_s": total * steps / wall,
+                      "gpu_ms_per_step": ms_graph / steps, "sane": sane}
+    log(f"[bench] sweep batched: {1e3 * wall / steps:.3f} ms/step")
+
+    for label, flags in (("per_flock", 0), ("per_flock_graph", E.FLAG_GRAPH)):
+        engines = []
+        for j, st in zip(jobs, states):
+            e = E.Engine(j.params(E.MODE_GRID_MIXED, flags, block=block))
+            e.set_state(xyzv6=st)
+            engines.append(e)
+        for _ in range(max(1, warmup // 5)):
+            for e in engines:
+                e.step(1)
+        for e in engines:
+            e.sync()
+        t0 = time.perf_counter()
+        for _ in range(base_steps):
+            for e in engines:
+                e.step(1)
+        for e in engines:
+            e.sync()
+        wall = time.perf_counter() - t0
+        for e in engines:
+            e.close()
+        out[label] = {"steps": base_steps, "ms_per_step": 1e3 * wall / base_steps,
+                      "boid_steps_per_s": total * base_steps / wall}
+        log(f"[bench] sweep {label}: {1e3 * wall / base_steps:.3f} ms/step")
+    out["speedup_vs_per_flock"] = out["batched"]["boid_steps_per_s"] / out["per_flock"]["boid_steps_per_s"]
+    out["speedup_vs_per_flock_graph"] = out["batched"]["boid_steps_per_s"] / out["per_flock_graph"]["boid_steps_per_s"]
+    return out
 
 
 def cpu_baseline(init, n, field, budget_s):
@@ -113,6 +187,10 @@ def main():
     ap.add_argument("--modes", default="grid_f64,grid_mixed")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="neighbour-kernel workgroup size (0: automatic)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
+    ap.add_argument("--sweep-steps", type=int, default=100, help="batched sweep steps (0: skip the sweep)")
+    ap.add_argument("--sweep-base-steps", type=int, default=10, help="steps of the per-flock baseline")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the path's "
@@ -126,7 +204,9 @@ def main():
     res = {}
     for name in a.modes.split(","):
         log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} steps={a.steps}")
-        res[name] = run_mode(by_name[name], init, n, F, a.steps, a.warmup)
+        params, tuning = tuned_params(by_name[name], init, n, F, not a.no_tune, a.block)
+        res[name] = run_mode(by_name[name], init, n, F, a.steps, a.warmup, params)
+        res[name]["tuning"] = tuning
         log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
 
     head = res.get("grid_mixed") or next(iter(res.values()))
@@ -135,6 +215,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(head["mode"], n),
                 "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": BYTES_PER_BOID_STEP}
+    sweep = None
+    if a.sweep_steps > 0:
+        log(f"[bench] sweep: 512 flocks, {a.sweep_steps} batched steps, {a.sweep_base_steps} per-flock steps")
+        sweep = run_sweep(a.sweep_steps, a.sweep_base_steps, 5, block=a.block)
     cpu = None
     if not a.no_cpu:
         log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s)")
@@ -162,6 +246,7 @@ def main():
                             if "grid_mixed" in res and "grid_f64" in res else None),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 

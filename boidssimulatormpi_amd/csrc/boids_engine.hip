@@ -39,6 +39,7 @@ struct bs_engine {
     // grid
     uint32_t *key = nullptr, *key_sorted = nullptr, *count = nullptr, *start = nullptr;
     uint32_t *partials = nullptr, *nchunks = nullptr, *perm = nullptr, *permid = nullptr;
+    uint32_t* rank = nullptr;          // arrival rank of each slot in its new cell
     uint4* table = nullptr;
     uint32_t max_chunks = 0, ncell1 = 0;
     // flock table (one entry for a single-flock engine)
@@ -47,12 +48,18 @@ struct bs_engine {
     FlockDesc* flocks_dev = nullptr;
     uint32_t* flock_of = nullptr;      // slot -> flock (batches only)
     uint32_t rows = 0;
-    int max_g = 0;
+    uint64_t groups = 0;               // (row, span group) pairs over all flocks
+    int max_g = 0, max_span = 0;
+    int nb = 256;                      // neighbour-kernel workgroup size
+    int tile = 9;                      // LDS tile depth (candidates per thread and pass)
+    int fill = 100;                    // chunk size target, percent of nb
+    uint32_t chunk = 256;              // boids per chunk at most (fill% of nb)
     bool batch = false;
     // one captured step (BS_FLAG_GRAPH)
     hipGraphExec_t graph = nullptr;
     bool capturing = false;
     Lattice lat{};
+    double2* rec = nullptr;            // FP64 grid: interleaved pair records
     uint32_t* recheck = nullptr;
     // outputs
     void* forces = nullptr;   // float or double, 9n, id order
@@ -502,8 +509,9 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         const bs_params& q = fp[k];
         const std::string at = batch ? "flock " + std::to_string(k) + ": " : std::string();
         if (!(q.field > 0.0) || !isfinite(q.field)) return fail(nullptr, BS_E_ARG, at + "field size must be positive and finite");
-        if (q.mode != p->mode || q.device != p->device || q.flags != p->flags)
-            return fail(nullptr, BS_E_ARG, at + "mode, device and flags must be the same for every flock");
+        if (q.mode != p->mode || q.device != p->device || q.flags != p->flags || q.block != p->block ||
+            q.tile != p->tile || q.fill != p->fill)
+            return fail(nullptr, BS_E_ARG, at + "mode, device, flags, block, tile and fill must be the same for every flock");
         ntot += q.n;
     }
     if (ntot > (1u << 28)) return fail(nullptr, BS_E_ARG, "n too large (max 2^28 boids per engine)");
@@ -519,6 +527,13 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     e->P.n = p->n;
 
     if (is_grid(e->mode)) {
+        if (p->block != 0 && p->block != 128 && p->block != 256) { release(e); return fail(nullptr, BS_E_ARG, "block must be 0, 128 or 256"); }
+        e->nb = p->block ? p->block : 256;
+        if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
+        if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
+        e->tile = p->tile ? p->tile : 9;
+        e->fill = p->fill ? p->fill : 100;
+        e->chunk = std::max(1u, (uint32_t)(e->fill * e->nb / 100));
         uint64_t cells = 0, rows = 0;
         bool coarse = false;
         for (uint32_t k = 0; k < count; ++k) {
@@ -540,6 +555,11 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             d.cell_base = (uint32_t)cells;
             d.row_base = (uint32_t)rows;
             d.nrows = (uint32_t)G * (uint32_t)G;
+            // rows are cut into the fewest equal groups of <= kSpanMax cells
+            const uint32_t ng = (uint32_t)((G + kSpanMax - 1) / kSpanMax);
+            d.span = (uint32_t)((G + ng - 1) / ng);
+            e->groups += (uint64_t)G * G * ((G + d.span - 1) / d.span);
+            e->max_span = std::max(e->max_span, (int)d.span);
             cells += (uint64_t)G * G * G;
             rows += (uint64_t)G * G;
             if (!(d.P.unit * 8.0 < 1e-6)) coarse = true;   // lattice too coarse for the mixed guard bands
@@ -583,13 +603,14 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     if ((rc = dev_alloc(e, &e->tmp_pos, 3 * (size_t)n)) || (rc = dev_alloc(e, &e->tmp_vel, 3 * (size_t)n))) return bail(rc);
     if (p->flags & BS_FLAG_DIGEST) { if ((rc = dev_alloc(e, &e->digest, 4 * (size_t)n))) return bail(rc); }
     if (is_grid(e->mode)) {
-        // sum over rows of ceil(n_row / kNbr) <= n / kNbr + rows
-        e->max_chunks = (n + kNbr - 1) / kNbr + e->rows + 1;
+        // sum over groups of ceil(n_group / chunk) <= n / chunk + groups
+        if ((uint64_t)(n + e->chunk - 1) / e->chunk + e->groups + 1 > 0x7fffffffull) return bail(fail(e, BS_E_ARG, "chunk table too large"));
+        e->max_chunks = (uint32_t)((n + e->chunk - 1) / e->chunk + e->groups + 1);
         const uint32_t ntiles = (e->ncell1 + 4095) / 4096;
         if ((rc = dev_alloc(e, &e->key, n)) || (rc = dev_alloc(e, &e->key_sorted, n)) ||
             (rc = dev_alloc(e, &e->count, e->ncell1)) || (rc = dev_alloc(e, &e->start, e->ncell1)) ||
             (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->nchunks, 1)) ||
-            (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) ||
+            (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) || (rc = dev_alloc(e, &e->rank, n)) ||
             (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)) ||
             (rc = dev_alloc(e, &e->flocks_dev, e->flocks.size())))
             return bail(rc);
@@ -604,9 +625,10 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         st = hipMemcpy(e->flocks_dev, e->flocks.data(), sizeof(FlockDesc) * e->flocks.size(), hipMemcpyHostToDevice);
         if (st != hipSuccess) { hip_fail(e, st, "hipMemcpy"); return bail(BS_E_HIP); }
         if (e->mode == BS_MODE_GRID_MIXED) {
-            if ((rc = dev_alloc(e, &e->lat.p, n)) || (rc = dev_alloc(e, &e->lat.vx, n)) ||
-                (rc = dev_alloc(e, &e->lat.vy, n)) || (rc = dev_alloc(e, &e->lat.vz, n)))
+            if ((rc = dev_alloc(e, &e->lat.p, n)) || (rc = dev_alloc(e, &e->lat.v, n)))
                 return bail(rc);
+        } else if ((rc = dev_alloc(e, &e->rec, 3 * (size_t)n))) {
+            return bail(rc);
         }
         st = hipMemsetAsync(e->count, 0, sizeof(uint32_t) * e->ncell1, e->stream);
         if (st == hipSuccess) st = hipMemsetAsync(e->recheck, 0, sizeof(uint32_t), e->stream);
@@ -688,14 +710,15 @@ static int grid_sort_into_a(bs_engine* e)
 {
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
-    HIP_TRY(e, launch_rows(flock_view(e), e->start, e->table, e->nchunks, e->stream));
+    HIP_TRY(e, launch_rows(flock_view(e), e->start, e->table, e->nchunks, (int)e->chunk, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
-    HIP_TRY(e, launch_scatter(e->prm.n, e->key, e->start, e->count, e->db.id, e->perm, e->permid, e->stream));
+    HIP_TRY(e, launch_scatter_ranked(e->prm.n, e->key, e->rank, e->start, e->db.id, e->perm, e->permid, e->stream));
     t2.stop();
     Timed t3(e, BS_K_GATHER);
     HIP_TRY(e, launch_gather(flock_view(e), e->perm, e->permid, e->key, e->start, e->count, e->db, e->da,
-                             e->key_sorted, e->mode == BS_MODE_GRID_MIXED ? &e->lat : nullptr, e->stream));
+                             e->key_sorted, e->mode == BS_MODE_GRID_MIXED ? &e->lat : nullptr,
+                             e->mode == BS_MODE_GRID_F64 ? e->rec : nullptr, e->stream));
     t3.stop();
     return BS_OK;
 }
@@ -722,7 +745,7 @@ static int upload_f64(bs_engine* e, const double* pos3, const double* vel3)
     HIP_TRY(e, hipMemcpyAsync(dst.vz, vz, b, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(dst.id, ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, e->stream));
     if (grid) {
-        HIP_TRY(e, launch_hash_count(flock_view(e), e->db, e->key, e->count, e->stream));
+        HIP_TRY(e, launch_hash_count(flock_view(e), e->db, e->key, e->rank, e->count, e->stream));
         int rc = grid_sort_into_a(e);
         if (rc) return rc;
     }
@@ -870,11 +893,11 @@ static int grid_step(bs_engine* e)
     Timed t(e, BS_K_PAIR);
     const Flocks F = flock_view(e);
     if (e->mode == BS_MODE_GRID_F64)
-        HIP_TRY(e, launch_nbr_f64(F, e->max_g, e->da, e->db, e->key_sorted, e->start, e->table, e->nchunks,
-                                  e->max_chunks, e->key, e->count, (double*)e->forces, e->digest, e->stream));
+        HIP_TRY(e, launch_nbr_f64(F, e->nb, e->tile, e->max_span, e->da, e->rec, e->db, e->key_sorted, e->start, e->table, e->nchunks,
+                                  e->max_chunks, e->key, e->rank, e->count, (double*)e->forces, e->digest, e->stream));
     else
-        HIP_TRY(e, launch_nbr_mixed(F, e->max_g, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
-                                    e->nchunks, e->max_chunks, e->key, e->count, (double*)e->forces,
+        HIP_TRY(e, launch_nbr_mixed(F, e->nb, e->tile, e->max_span, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
+                                    e->nchunks, e->max_chunks, e->key, e->rank, e->count, (double*)e->forces,
                                     e->digest, e->recheck, e->stream));
     t.stop();
     return grid_sort_into_a(e);
@@ -959,6 +982,10 @@ extern "C" int bs_get_info(const bs_engine* e, bs_info* out)
     out->cell_edge = e->edge;
     out->device_bytes = e->device_bytes;
     out->steps = e->steps;
+    out->block = is_grid(e->mode) ? e->nb : 0;
+    out->tile = is_grid(e->mode) ? e->tile : 0;
+    out->fill = is_grid(e->mode) ? e->fill : 0;
+    out->reserved = 0;
     return BS_OK;
 }
 

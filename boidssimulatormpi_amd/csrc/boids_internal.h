@@ -11,22 +11,11 @@ namespace boids {
 enum { SEP = 0, ALI = 1, COH = 2 };
 
 constexpr int kBlock = 256;        // threads per block of the streaming kernels
-#ifndef BS_NBR_BLOCK
-#define BS_NBR_BLOCK 256
-#endif
-constexpr int kNbr = BS_NBR_BLOCK;  // threads (= boids per chunk) of the neighbour kernels
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
-// LDS candidate capacity per pass; a row segment of a kNbr-boid chunk holds
-// about kNbr + 2 cells of candidates, larger segments take several passes.
-constexpr int kCapF64 = kNbr + 128;
-constexpr int kCapMixed = kNbr + 256;
-#ifndef BS_QUEUE_DEPTH
-#define BS_QUEUE_DEPTH 16
-#endif
-#ifndef BS_NBR_QUEUE
-#define BS_NBR_QUEUE 1             // 1: filter -> per-lane queue -> dense pass; 0: evaluate inline
-#endif
-constexpr int kQueue = BS_QUEUE_DEPTH;   // per-lane queue of in-range candidates (LDS indices)
+// Neighbour kernels: a workgroup of NB threads (128 or 256) owns a chunk of
+// at most NB sorted boids inside one (y,z) row of cells, spanning at most
+// kSpanMax cells of that row.
+constexpr int kSpanMax = 62;
 #ifndef BS_NBR_WAVES
 #define BS_NBR_WAVES 0             // >0: ask the compiler for at least this many waves per SIMD
 #endif
@@ -76,8 +65,8 @@ struct StateF32 {
 };
 
 struct Lattice {               // mixed mode companion of the sorted FP64 state
-    uint4* p;                  // x, y, z lattice coordinates, w = sorted slot
-    float *vx, *vy, *vz;       // FP32 copies of the velocity
+    uint4* p;                  // x, y, z lattice coordinates (w unused)
+    float4* v;                 // FP32 copies of the velocity (w unused)
 };
 
 // One flock of a grid engine.  A grid engine simulates one or more
@@ -88,6 +77,8 @@ struct Lattice {               // mixed mode companion of the sorted FP64 state
 struct FlockDesc {
     DevParams P;
     uint32_t boid_base, cell_base, row_base, nrows;
+    uint32_t span;             // cells per neighbour-kernel chunk group (<= kSpanMax)
+    uint32_t pad[3];
 };
 
 // Kernel view of the flock table.
@@ -104,23 +95,32 @@ hipError_t launch_allpairs_f32ref(const DevParams& P, StateF32 in, StateF32 out,
                                   uint64_t* digest, hipStream_t stream);
 hipError_t launch_allpairs_f64(const DevParams& P, StateF64 in, StateF64 out, double* forces,
                                uint64_t* digest, hipStream_t stream);
-hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* count, hipStream_t stream);
+hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* rank, uint32_t* count,
+                             hipStream_t stream);
 hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
                        hipStream_t stream);
-hipError_t launch_rows(Flocks F, const uint32_t* start, uint4* table, uint32_t* nchunks,
+hipError_t launch_rows(Flocks F, const uint32_t* start, uint4* table, uint32_t* nchunks, int chunk,
                        hipStream_t stream);
 hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
                           const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
+// Scatter by the arrival ranks the counting pass recorded (no atomics).
+hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
+                                 const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
+// lat (mixed) receives the lattice copy; rec (FP64 grid) the interleaved
+// records x y z vx vy vz (3 double2 per slot) the neighbour kernel fetches.
 hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,
                          const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
-                         uint32_t* key_sorted, Lattice* lat, hipStream_t stream);
-hipError_t launch_nbr_f64(Flocks F, int max_g, StateF64 in, StateF64 out, const uint32_t* key_sorted,
-                          const uint32_t* start, const uint4* table, const uint32_t* nchunks,
-                          uint32_t max_chunks, uint32_t* key_out, uint32_t* count, double* forces,
-                          uint64_t* digest, hipStream_t stream);
-hipError_t launch_nbr_mixed(Flocks F, int max_g, StateF64 in, StateF64 out, Lattice lat,
+                         uint32_t* key_sorted, Lattice* lat, double2* rec, hipStream_t stream);
+// Neighbour kernels (boids_nbr.hip).  nb = workgroup size (128 or 256),
+// tile = LDS candidates per thread and pass (6, 9 or 12), max_span =
+// largest FlockDesc::span, max_chunks = capacity of `table`.
+hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in, const double2* rec,
+                          StateF64 out, const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
+                          const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
+                          uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream);
+hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_span, StateF64 in, StateF64 out, Lattice lat,
                             const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
-                            const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out,
+                            const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
                             hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);

@@ -1,0 +1,570 @@
+// boids_nbr.hip — fused neighbour + integrate kernels of the grid modes
+// (the cell-list update of src/boid_simulation_grid.cpp:165-293, phase A of
+// src/boid_simulation.cpp:153-212 per pair, phase B :216-248).
+//
+// A workgroup of NB threads owns a chunk [t0, t1) of sorted boids inside
+// one (y,z) row of cells, spanning cells cxa..cxb of that row (at most
+// kSpanMax).  Its tile is the 3 x 3 neighbour rows over the columns
+// cxa-1 .. cxb+1, ordered column-major: entry e = 9 * column + rr, with
+// rr = 3 * (dz + 1) + (dy + 1), each entry the boids of one cell in sorted
+// order.  A boid in column L therefore finds its whole 27-cell window in the
+// one contiguous tile range [off(L-1, 0), off(L+2, 0)).
+//
+// Per candidate the tile holds a coarse position (top 16 bits of the 32-bit
+// lattice coordinate per axis) and the sorted slot.  Phase 1 tests every
+// candidate of the window with a packed 16-bit squared distance that never
+// drops a pair inside the largest radius, and appends hits to a per-lane LDS
+// queue; phase 2 drains the queue densely, fetching the candidates' full
+// records from L2 by slot, and takes the rule decisions:
+//   FP64 plain: the oracle's expressions (exact thresholds, s < s_thr and
+//               c >= c_thr && c <= 1, on the minimum image);
+//   mixed:      FP32 on exact lattice differences outside guard bands,
+//               the FP64 expression inside them.
+// Tiles larger than the LDS capacity are processed in passes.
+#include "boids_device.h"
+
+namespace boids {
+
+template <int NB, int TD> struct NbrCfg {
+    static constexpr int kCap = NB * TD;    // staged candidates per pass
+    static constexpr int kQueue = 16;       // per-lane queue depth (tile indices)
+};
+
+// Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
+// sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg.
+__host__ __device__ constexpr size_t tile_table_bytes(int max_span)
+{
+    return ((size_t)(9 * (max_span + 2) + 1) * 4 + (size_t)(9 * (max_span + 2)) * 4 + 15) & ~(size_t)15;
+}
+
+// Builds the entry tables of the chunk's tile (counts, then a block-wide
+// exclusive scan; each thread owns consecutive entries).
+template <int NB>
+__device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ start, uint32_t cbase, int G,
+                                                  int cxa, int nseg, int cy, int cz, uint32_t* off,
+                                                  uint32_t* gst, uint32_t* wsum)
+{
+    const int E = 9 * nseg;
+    const int per = (E + NB - 1) / NB;
+    const int e0 = threadIdx.x * per;
+    const int e1 = min(E, e0 + per);
+    uint32_t local = 0;
+    for (int e = e0; e < e1; ++e) {
+        const int xc = e / 9, rr = e - 9 * xc;
+        int cx = cxa - 1 + xc;
+        if (cx < 0) cx += G; else if (cx >= G) cx -= G;
+        int ny = cy + rr % 3 - 1, nz = cz + rr / 3 - 1;
+        if (ny < 0) ny += G; else if (ny >= G) ny -= G;
+        if (nz < 0) nz += G; else if (nz >= G) nz -= G;
+        const uint32_t c = cbase + (uint32_t)cx + (uint32_t)G * ((uint32_t)ny + (uint32_t)G * (uint32_t)nz);
+        const uint32_t a = start[c], b = start[c + 1];
+        gst[e] = a;
+        off[e] = b - a;                 // the count, until the scan below
+        local += b - a;
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(local, wsum, &total);
+    for (int e = e0; e < e1; ++e) {
+        const uint32_t cnt = off[e];
+        off[e] = run;
+        run += cnt;
+    }
+    if (threadIdx.x == 0) off[E] = total;
+    __syncthreads();
+}
+
+// Coarse position of a lattice point: top 16 bits of x|y, z.
+__device__ __forceinline__ uint2 coarse_of(uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    return make_uint2((lx >> 16) | (ly & 0xFFFF0000u), lz >> 16);
+}
+
+// Squared coarse distance (packed int16 minimum-image differences; the
+// dot products saturate instead of wrapping).
+__device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, v2s my_z)
+{
+    const v2s dxy = __builtin_bit_cast(v2s, q.x) - my_xy;
+    const v2s dzz = __builtin_bit_cast(v2s, q.y) - my_z;
+    return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dzz, dzz, 0, true), true);
+}
+
+// Stages tile positions [base, base + here) (coarse position + slot); each
+// thread copies one run of consecutive positions.  POS(slot) returns the
+// candidate's coarse position.
+template <int NB, typename POS>
+__device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* gst, int E, uint32_t base,
+                                           uint32_t here, uint2* tco, uint32_t* tsl, POS pos)
+{
+    const uint32_t per = (here + NB - 1) / NB;
+    const uint32_t p0 = base + threadIdx.x * per;
+    const uint32_t p1 = min(base + here, p0 + per);
+    if (p0 >= p1) return;
+    int lo = 0, hi = E - 1;                  // last entry with off <= p0
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= p0) lo = mid; else hi = mid - 1;
+    }
+    int e = lo;
+    uint32_t eb = off[e], ee = off[e + 1], g = gst[e];
+    for (uint32_t p = p0; p < p1; ++p) {
+        while (p >= ee) { ++e; eb = ee; ee = off[e + 1]; g = gst[e]; }
+        const uint32_t slot = g + (p - eb);
+        tco[p - base] = pos(slot);
+        tsl[p - base] = slot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 plain.
+// ---------------------------------------------------------------------------
+template <int NB, int TD, bool DIGEST>
+__global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, StateF64 in,
+                                                    const double2* __restrict__ rec, StateF64 out,
+                                                    const uint32_t* __restrict__ key_sorted,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint4* __restrict__ table,
+                                                    const uint32_t* __restrict__ nchunks,
+                                                    uint32_t* key_out, uint32_t* rank, uint32_t* count,
+                                                    double* forces, uint64_t* digest)
+{
+    using Cfg = NbrCfg<NB, TD>;
+    __shared__ uint2 tco[Cfg::kCap];
+    __shared__ uint32_t tsl[Cfg::kCap];
+    __shared__ uint16_t sq[Cfg::kQueue * NB];   // per-lane queues, lane-interleaved
+    __shared__ uint32_t wsum[17];
+    extern __shared__ __align__(16) unsigned char dyn[];
+
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= *nchunks) return;
+    const uint4 ch = table[b];
+    const uint32_t row = __builtin_amdgcn_readfirstlane(ch.x);
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(ch.y);
+    const uint32_t t1 = __builtin_amdgcn_readfirstlane(ch.z);
+    const uint32_t f = __builtin_amdgcn_readfirstlane(ch.w);
+    const DevParams P = FL.fl[f].P;
+    const uint32_t cbase = FL.fl[f].cell_base;
+    const int G = P.G;
+    const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
+    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
+    const int nseg = cxb - cxa + 3;
+    const int E = 9 * nseg;
+    uint32_t* off = (uint32_t*)dyn;
+    uint32_t* gst = off + (E + 1);
+
+    const uint32_t s = t0 + threadIdx.x;
+    const bool active = s < t1;
+    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t myid = 0, mystart = 0;
+    int L = 1;
+    if (active) {
+        px = in.x[s]; py = in.y[s]; pz = in.z[s];
+        vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
+        myid = in.id[s];
+        const uint32_t mykey = key_sorted[s];
+        mystart = start[mykey];
+        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
+    }
+    const double sc = P.lat_scale;
+    const uint2 mc = coarse_of(to_lattice(px, sc), to_lattice(py, sc), to_lattice(pz, sc));
+    const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
+    const v2s my_z = __builtin_bit_cast(v2s, mc.y);
+    const double vn = norm3(vx, vy, vz);
+    const double F = P.F, half = P.half;
+    const bool k_c = P.cone[COH], k_s = P.cone[SEP], k_a = P.cone[ALI];
+
+    double scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
+    int nc = 0, ns = 0, na = 0;
+    uint64_t hc = 0, hs = 0, ha = 0;
+
+    build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
+    const uint32_t total = off[E];
+    const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
+    const uint32_t selfk = off[9 * L + 4] + (s - mystart);
+    uint32_t selfk_t = 0xFFFFFFFFu;            // own tile index in the current pass
+
+    // One queued candidate: its record, fetched from L2 by slot; `skip`
+    // marks the boid itself and the padding entry of an odd queue.
+    struct Rec { double x, y, z, vx, vy, vz; uint32_t id; bool skip; };
+    auto fetch = [&](uint32_t kk, bool pad) {
+        Rec r;
+        const uint32_t j = tsl[kk];
+        const double2 a = rec[3 * (size_t)j], b = rec[3 * (size_t)j + 1], c = rec[3 * (size_t)j + 2];
+        r.x = a.x; r.y = a.y; r.z = b.x;
+        r.vx = b.y; r.vy = c.x; r.vz = c.y;
+        r.id = DIGEST ? in.id[j] : 0u;
+        r.skip = pad || kk == selfk_t;
+        return r;
+    };
+    // The oracle's decisions on the minimum image; branch-free sums (an
+    // fma with a 0/1 factor adds q or leaves the sum unchanged, exactly as
+    // the oracle's conditional add does) so that consecutive pairs overlap.
+    auto eval = [&](const Rec& r) {
+        const double qx = min_image(r.x, px, F, half);
+        const double qy = min_image(r.y, py, F, half);
+        const double qz = min_image(r.z, pz, F, half);
+        const double ex = px - qx, ey = py - qy, ez = pz - qz;
+        const double s2 = ex * ex + ey * ey + ez * ez;
+        const double gx = qx - px, gy = qy - py, gz = qz - pz;
+        const double dot = vx * gx + vy * gy + vz * gz;
+        const double t = sqrt(s2);
+        const double c = dot / (vn * t);
+        const bool cin = c <= 1.0;
+        const bool in_c = !r.skip && s2 < P.s_thr[COH] && (!k_c || (c >= P.c_thr[COH] && cin));
+        const bool in_s = !r.skip && s2 < P.s_thr[SEP] && (!k_s || (c >= P.c_thr[SEP] && cin));
+        const bool in_a = !r.skip && s2 < P.s_thr[ALI] && (!k_a || (c >= P.c_thr[ALI] && cin));
+        const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
+        nc += in_c; scx = fma(qx, mc, scx); scy = fma(qy, mc, scy); scz = fma(qz, mc, scz);
+        na += in_a; sax = fma(r.vx, ma, sax); say = fma(r.vy, ma, say); saz = fma(r.vz, ma, saz);
+        if (in_s) { ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t; }
+        if (DIGEST) {
+            const uint64_t h = mix64(r.id);
+            hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
+        }
+    };
+    // Pairs two at a time, the next two fetched while these are evaluated
+    // (indices clamped to the queue, the padding entry skipped).
+    auto drain = [&](uint32_t qn) {
+        if (qn == 0) return;
+        const uint32_t last = qn - 1;
+        Rec c0 = fetch(sq[threadIdx.x], false);
+        Rec c1 = fetch(sq[min(1u, last) * NB + threadIdx.x], 1 > last);
+        for (uint32_t j = 0; j < qn; j += 2) {
+            const Rec n0 = fetch(sq[min(j + 2, last) * NB + threadIdx.x], j + 2 > last);
+            const Rec n1 = fetch(sq[min(j + 3, last) * NB + threadIdx.x], j + 3 > last);
+            eval(c0);
+            eval(c1);
+            c0 = n0;
+            c1 = n1;
+        }
+    };
+
+    for (uint32_t base = 0; base < total; base += Cfg::kCap) {
+        const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
+        __syncthreads();                       // previous readers of the tile are done
+        stage_tile<NB>(off, gst, E, base, here, tco, tsl, [&](uint32_t slot) {
+            return coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
+        });
+        __syncthreads();
+        if (!active) continue;
+        selfk_t = selfk - base;                // wraps past the pass when not in it
+        const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
+        auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
+        uint32_t qn = 0, k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            if (__any(qn > (uint32_t)Cfg::kQueue - 4)) { drain(qn); qn = 0; }
+            const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
+            const uint32_t kk = k - base;
+            sq[qn * NB + threadIdx.x] = (uint16_t)kk; qn += h0;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+        }
+        for (; k < ke; ++k) {
+            if (__any(qn == (uint32_t)Cfg::kQueue)) { drain(qn); qn = 0; }
+            sq[qn * NB + threadIdx.x] = (uint16_t)(k - base);
+            qn += near(k);
+        }
+        drain(qn);
+    }
+    if (!active) return;
+    if (nc != 0) { const double k = (double)nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
+    if (na != 0) { const double k = (double)na; sax = sax / k - vx; say = say / k - vy; saz = saz / k - vz; }
+    if (DIGEST) {
+        uint64_t* d = digest + 4 * (size_t)myid;
+        d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
+        d[1] = hc; d[2] = hs; d[3] = ha;
+    }
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, scx, scy, scz, ssx, ssy, ssz, sax, say, saz,
+                out, key_out, rank, count, forces);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision.  The difference of two lattice coordinates (unit F/2^32),
+// read as int32, is the minimum-image displacement exactly, so the FP32 pair
+// test works on exact integers.  A decision is taken in FP32 when it lies
+// outside its guard band and re-taken with the oracle's FP64 expression
+// otherwise.  Sums are FP64 (cohesion: exact integer lattice sums); phase B
+// runs in FP64.
+// ---------------------------------------------------------------------------
+// FP32 verdict of one rule: in = surely inside, out = surely outside
+// (neither: inside a guard band, decided in FP64).  c_dn / c_up are the
+// FP32 cosine moved down / up by the displacement band; c_ok = cosine not
+// within 1e-5 of 1.  A skipped candidate is surely outside every rule.
+__device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, float c_dn, float c_up, bool c_ok,
+                                           bool skip, bool& in, bool& out)
+{
+    const bool s_in = s2 < P.s_lo[r];
+    const bool s_out = !(s2 < P.s_hi[r]);
+    if (P.cone[r]) {
+        in = s_in && c_dn >= P.c_hi[r] && c_ok;
+        out = s_out || c_up <= P.c_lo[r];
+    } else {
+        in = s_in;
+        out = s_out;
+    }
+    in = in && !skip;
+    out = out || skip;
+}
+
+template <int NB, int TD, bool DIGEST>
+__global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, StateF64 in, StateF64 out,
+                                                      Lattice lat,
+                                                      const uint32_t* __restrict__ key_sorted,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint4* __restrict__ table,
+                                                      const uint32_t* __restrict__ nchunks,
+                                                      uint32_t* key_out, uint32_t* rank, uint32_t* count,
+                                                      double* forces, uint64_t* digest, uint32_t* recheck)
+{
+    using Cfg = NbrCfg<NB, TD>;
+    __shared__ uint2 tco[Cfg::kCap];
+    __shared__ uint32_t tsl[Cfg::kCap];
+    __shared__ uint16_t sq[Cfg::kQueue * NB];
+    __shared__ uint32_t wsum[17];
+    extern __shared__ __align__(16) unsigned char dyn[];
+
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= *nchunks) return;
+    const uint4 ch = table[b];
+    const uint32_t row = __builtin_amdgcn_readfirstlane(ch.x);
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(ch.y);
+    const uint32_t t1 = __builtin_amdgcn_readfirstlane(ch.z);
+    const uint32_t f = __builtin_amdgcn_readfirstlane(ch.w);
+    const DevParams P = FL.fl[f].P;
+    const uint32_t cbase = FL.fl[f].cell_base;
+    const int G = P.G;
+    const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
+    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
+    const int nseg = cxb - cxa + 3;
+    const int E = 9 * nseg;
+    uint32_t* off = (uint32_t*)dyn;
+    uint32_t* gst = off + (E + 1);
+
+    const uint32_t s = t0 + threadIdx.x;
+    const bool active = s < t1;
+    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t myid = 0, mystart = 0;
+    uint4 me = make_uint4(0, 0, 0, 0);
+    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    int L = 1;
+    if (active) {
+        px = in.x[s]; py = in.y[s]; pz = in.z[s];
+        vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
+        myid = in.id[s];
+        me = lat.p[s];
+        const float4 fv = lat.v[s];
+        fvx = fv.x; fvy = fv.y; fvz = fv.z;
+        const uint32_t mykey = key_sorted[s];
+        mystart = start[mykey];
+        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
+    }
+    const uint2 mc = coarse_of(me.x, me.y, me.z);
+    const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
+    const v2s my_z = __builtin_bit_cast(v2s, mc.y);
+    const double vn = norm3(vx, vy, vz);
+    const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
+    // a vanishing FP32 velocity makes the FP32 cosine meaningless: recheck
+    const bool vbad = !(fvv > 1e-30f);
+    const float inv_vn = vbad ? 0.f : __builtin_amdgcn_rsqf(fvv);
+
+    double scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
+    int nc = 0, ns = 0, na = 0;
+    uint32_t nre = 0;
+    uint64_t hc = 0, hs = 0, ha = 0;
+
+    build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
+    const uint32_t total = off[E];
+    const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
+    const uint32_t selfk = off[9 * L + 4] + (s - mystart);
+    uint32_t selfk_t = 0xFFFFFFFFu;            // own tile index in the current pass
+
+    struct Rec { uint4 q; float4 w; uint32_t j; bool skip; };
+    auto fetch = [&](uint32_t kk, bool pad) {
+        Rec r;
+        r.j = tsl[kk];
+        r.q = lat.p[r.j];
+        r.w = lat.v[r.j];
+        r.skip = pad || kk == selfk_t;
+        return r;
+    };
+    auto eval = [&](const Rec& r) {
+        const int32_t ix = (int32_t)(r.q.x - me.x), iy = (int32_t)(r.q.y - me.y), iz = (int32_t)(r.q.z - me.z);
+        const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
+        const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+        const float inv_d = __builtin_amdgcn_rsqf(s2);
+        const float c = fmaf(fvz, gz, fmaf(fvy, gy, fvx * gx)) * inv_d * inv_vn;
+        const float band = P.band_k * inv_d;
+        const float c_dn = c - band, c_up = c + band;
+        const bool c_ok = c < P.c_top;
+        bool in_c, out_c, in_s, out_s, in_a, out_a;
+        rule_bands(P, COH, s2, c_dn, c_up, c_ok, r.skip, in_c, out_c);
+        rule_bands(P, SEP, s2, c_dn, c_up, c_ok, r.skip, in_s, out_s);
+        rule_bands(P, ALI, s2, c_dn, c_up, c_ok, r.skip, in_a, out_a);
+        const bool sure = (in_c || out_c) && (in_s || out_s) && (in_a || out_a) &&
+                          !(vbad && !(out_c && out_s && out_a));
+        if (!sure) {
+            exact_decide(P, px, py, pz, vx, vy, vz, vn, in.x[r.j], in.y[r.j], in.z[r.j], in_s, in_a, in_c);
+            ++nre;
+        }
+        // 0/1 factors on finite terms (integer displacements, velocities);
+        // the separation term is selected first (it is not finite at d = 0)
+        const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
+        nc += in_c;
+        scx = fma((double)ix, mc, scx); scy = fma((double)iy, mc, scy); scz = fma((double)iz, mc, scz);
+        const float ts = in_s ? -inv_d : 0.f;
+        ns += in_s;
+        ssx += (double)(gx * ts); ssy += (double)(gy * ts); ssz += (double)(gz * ts);
+        na += in_a;
+        sax = fma((double)r.w.x, ma, sax); say = fma((double)r.w.y, ma, say); saz = fma((double)r.w.z, ma, saz);
+        if (DIGEST) {
+            const uint64_t h = mix64(in.id[r.j]);
+            hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
+        }
+    };
+    auto drain = [&](uint32_t qn) {
+        if (qn == 0) return;
+        const uint32_t last = qn - 1;
+        Rec c0 = fetch(sq[threadIdx.x], false);
+        Rec c1 = fetch(sq[min(1u, last) * NB + threadIdx.x], 1 > last);
+        for (uint32_t j = 0; j < qn; j += 2) {
+            const Rec n0 = fetch(sq[min(j + 2, last) * NB + threadIdx.x], j + 2 > last);
+            const Rec n1 = fetch(sq[min(j + 3, last) * NB + threadIdx.x], j + 3 > last);
+            eval(c0);
+            eval(c1);
+            c0 = n0;
+            c1 = n1;
+        }
+    };
+
+    for (uint32_t base = 0; base < total; base += Cfg::kCap) {
+        const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
+        __syncthreads();
+        stage_tile<NB>(off, gst, E, base, here, tco, tsl, [&](uint32_t slot) {
+            const uint4 q = lat.p[slot];
+            return coarse_of(q.x, q.y, q.z);
+        });
+        __syncthreads();
+        if (!active) continue;
+        selfk_t = selfk - base;
+        const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
+        auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
+        uint32_t qn = 0, k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            if (__any(qn > (uint32_t)Cfg::kQueue - 4)) { drain(qn); qn = 0; }
+            const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
+            const uint32_t kk = k - base;
+            sq[qn * NB + threadIdx.x] = (uint16_t)kk; qn += h0;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
+            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+        }
+        for (; k < ke; ++k) {
+            if (__any(qn == (uint32_t)Cfg::kQueue)) { drain(qn); qn = 0; }
+            sq[qn * NB + threadIdx.x] = (uint16_t)(k - base);
+            qn += near(k);
+        }
+        drain(qn);
+    }
+    if (nre) atomicAdd(recheck, nre);
+    if (!active) return;
+    // dv_coh = mean(q) - p = mean(q - p): the lattice sums are already relative
+    double dcx = 0, dcy = 0, dcz = 0, dax = 0, day = 0, daz = 0;
+    if (nc != 0) {
+        const double k = P.unit / (double)nc;
+        dcx = scx * k; dcy = scy * k; dcz = scz * k;
+    }
+    if (na != 0) {
+        const double k = (double)na;
+        dax = sax / k - vx; day = say / k - vy; daz = saz / k - vz;
+    }
+    if (DIGEST) {
+        uint64_t* d = digest + 4 * (size_t)myid;
+        d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
+        d[1] = hc; d[2] = hs; d[3] = ha;
+    }
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, ssx, ssy, ssz, dax, day, daz,
+                out, key_out, rank, count, forces);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t round8(uint32_t g) { return (g + 7u) & ~7u; }
+
+template <int NB, int TD>
+static void launch_f64_nb(Flocks F, size_t dyn, dim3 grid, StateF64 in, const double2* rec, StateF64 out,
+                          const uint32_t* key_sorted,
+                          const uint32_t* start, const uint4* table, const uint32_t* nchunks, uint32_t* key_out,
+                          uint32_t* rank, uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream)
+{
+    if (digest)
+        hipLaunchKernelGGL((k_nbr_f64<NB, TD, true>), grid, dim3(NB), dyn, stream, F, in, rec, out, key_sorted, start,
+                           table, nchunks, key_out, rank, count, forces, digest);
+    else
+        hipLaunchKernelGGL((k_nbr_f64<NB, TD, false>), grid, dim3(NB), dyn, stream, F, in, rec, out, key_sorted, start,
+                           table, nchunks, key_out, rank, count, forces, digest);
+}
+
+template <int NB, int TD>
+static void launch_mixed_nb(Flocks F, size_t dyn, dim3 grid, StateF64 in, StateF64 out, Lattice lat,
+                            const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
+                            const uint32_t* nchunks, uint32_t* key_out, uint32_t* rank, uint32_t* count,
+                            double* forces, uint64_t* digest, uint32_t* recheck, hipStream_t stream)
+{
+    if (digest)
+        hipLaunchKernelGGL((k_nbr_mixed<NB, TD, true>), grid, dim3(NB), dyn, stream, F, in, out, lat, key_sorted,
+                           start, table, nchunks, key_out, rank, count, forces, digest, recheck);
+    else
+        hipLaunchKernelGGL((k_nbr_mixed<NB, TD, false>), grid, dim3(NB), dyn, stream, F, in, out, lat, key_sorted,
+                           start, table, nchunks, key_out, rank, count, forces, digest, recheck);
+}
+
+// Instantiates LAUNCH<NB, TD> for the supported (workgroup size, tile depth)
+// pairs; false for any other pair.
+#define BS_NBR_DISPATCH(nb, tile, CALL)                                        \
+    do {                                                                      \
+        if (nb == 128 && tile == 6) { CALL(128, 6); break; }                  \
+        if (nb == 128 && tile == 9) { CALL(128, 9); break; }                  \
+        if (nb == 128 && tile == 12) { CALL(128, 12); break; }                \
+        if (nb == 256 && tile == 6) { CALL(256, 6); break; }                  \
+        if (nb == 256 && tile == 9) { CALL(256, 9); break; }                  \
+        if (nb == 256 && tile == 12) { CALL(256, 12); break; }                \
+        return hipErrorInvalidValue;                                          \
+    } while (0)
+
+hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in, const double2* rec,
+                          StateF64 out, const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
+                          const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
+                          uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream)
+{
+    if (F.n == 0) return hipSuccess;
+    if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
+    const dim3 grid(round8(max_chunks));
+    const size_t dyn = tile_table_bytes(max_span);
+#define BS_CALL(NB_, TD_) launch_f64_nb<NB_, TD_>(F, dyn, grid, in, rec, out, key_sorted, start, table, nchunks, \
+                                                   key_out, rank, count, forces, digest, stream)
+    BS_NBR_DISPATCH(nb, tile, BS_CALL);
+#undef BS_CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_span, StateF64 in, StateF64 out, Lattice lat,
+                            const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
+                            const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
+                            uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
+                            hipStream_t stream)
+{
+    if (F.n == 0) return hipSuccess;
+    if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
+    const dim3 grid(round8(max_chunks));
+    const size_t dyn = tile_table_bytes(max_span);
+#define BS_CALL(NB_, TD_) launch_mixed_nb<NB_, TD_>(F, dyn, grid, in, out, lat, key_sorted, start, table, nchunks, \
+                                                     key_out, rank, count, forces, digest, recheck, stream)
+    BS_NBR_DISPATCH(nb, tile, BS_CALL);
+#undef BS_CALL
+    return hipGetLastError();
+}
+
+}  // namespace boids

@@ -47,13 +47,15 @@ class Params(ctypes.Structure):
                 ("coh_r", ctypes.c_double), ("coh_a", ctypes.c_double), ("coh_w", ctypes.c_double),
                 ("vmax", ctypes.c_double), ("vmin", ctypes.c_double),
                 ("mode", ctypes.c_int32), ("cells_per_axis", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("block", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("fill", ctypes.c_int32)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("cells_per_axis", ctypes.c_int32),
                 ("cell_edge", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
-                ("steps", ctypes.c_uint64)]
+                ("steps", ctypes.c_uint64), ("block", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("fill", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class FlockInfo(ctypes.Structure):
@@ -142,8 +144,9 @@ def _p(a, ct):
 
 
 def make_params(n, field, sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008),
-                vmax=0.005, vmin=0.001, mode=MODE_GRID_F64, flags=0, cells_per_axis=0, device=0) -> Params:
-    return Params(n, flags, field, *sep, *ali, *coh, vmax, vmin, mode, cells_per_axis, device, 0)
+                vmax=0.005, vmin=0.001, mode=MODE_GRID_F64, flags=0, cells_per_axis=0, device=0, block=0,
+                tile=0, fill=0) -> Params:
+    return Params(n, flags, field, *sep, *ali, *coh, vmax, vmin, mode, cells_per_axis, device, block, tile, fill)
 
 
 def generate_init(kind: int, n: int, field: float, vmax: float, vmin: float, seed=None) -> np.ndarray:

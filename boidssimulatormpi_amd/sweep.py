@@ -54,9 +54,9 @@ class Job:
     setting: str = ""
     extra: dict = dfield(default_factory=dict)
 
-    def params(self, mode=E.MODE_GRID_MIXED, flags=0, device=0) -> E.Params:
+    def params(self, mode=E.MODE_GRID_MIXED, flags=0, device=0, block=0) -> E.Params:
         return E.make_params(self.n, self.field, self.sep, self.ali, self.coh, self.vmax, self.vmin,
-                             mode=mode, flags=flags, device=device)
+                             mode=mode, flags=flags, device=device, block=block)
 
 
 # ----------------------------------------------------------------------------
@@ -208,11 +208,11 @@ def initial_state(job: Job) -> np.ndarray:
 class Sweep:
     """A batched engine over a job list, plus per-job views of its arrays."""
 
-    def __init__(self, jobs, mode=E.MODE_GRID_MIXED, flags=E.FLAG_GRAPH, device=0):
+    def __init__(self, jobs, mode=E.MODE_GRID_MIXED, flags=E.FLAG_GRAPH, device=0, block=0):
         if not jobs:
             raise ValueError("empty sweep")
         self.jobs = list(jobs)
-        self.engine = E.Engine([j.params(mode, flags, device) for j in self.jobs])
+        self.engine = E.Engine([j.params(mode, flags, device, block) for j in self.jobs])
         self.base = np.cumsum([0] + [j.n for j in self.jobs])
 
     def set_initial_state(self, states=None):

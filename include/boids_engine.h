@@ -85,7 +85,11 @@ typedef struct {
     int32_t mode;                /* BS_MODE_* */
     int32_t cells_per_axis;      /* grid modes: 0 = automatic */
     int32_t device;              /* HIP device ordinal */
-    int32_t reserved;
+    int32_t block;               /* grid modes: neighbour-kernel workgroup size, 128 or 256; 0 = automatic */
+    int32_t tile;                /* grid modes: LDS tile depth, candidates staged per thread and pass:
+                                  * 6, 9 or 12; 0 = automatic (9) */
+    int32_t fill;                /* grid modes: most boids per chunk, percent of the workgroup size
+                                  * (25..100); 0 = automatic (100) */
 } bs_params;
 
 /* Read-only facts about a created engine. */
@@ -95,6 +99,10 @@ typedef struct {
     double cell_edge;
     uint64_t device_bytes;       /* device memory the engine holds */
     uint64_t steps;              /* updates done since the last set_state/init */
+    int32_t block;               /* neighbour-kernel workgroup size (grid modes), else 0 */
+    int32_t tile;                /* LDS tile depth (grid modes), else 0 */
+    int32_t fill;                /* chunk size limit in percent of block (grid modes), else 0 */
+    int32_t reserved;
 } bs_info;
 
 /* Kernel ids for bs_get_timing. */

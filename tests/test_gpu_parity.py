@@ -281,10 +281,13 @@ def test_metrics_match_oracle(mode):
         assert abs(g["mean_nn"] - o["mean_nn"]) < 1e-12 * F
 
 
-def test_long_run_statistics_fp64_vs_mixed_vs_oracle():
+def test_long_run_statistics_fp64_vs_mixed():
     # Chaotic runs diverge pointwise; their statistics must agree.  N = 2^20,
     # 600 steps on the GPU in both precisions; time-averaged over the last
-    # 100 steps (every 20).
+    # 100 steps (every 20).  With these rules the flock stays disordered: its
+    # polarisation is the random-direction value ~0.92/sqrt(N) (sd ~0.39/sqrt(N)),
+    # so polarisation is compared against that noise floor; the ordered case
+    # is test_long_run_polarisation_ordered_regime.
     n, F = 1 << 20, 4.0
     st = iso_state(n, F, 2)
     avg = {}
@@ -299,5 +302,40 @@ def test_long_run_statistics_fp64_vs_mixed_vs_oracle():
         avg[mode] = {k: np.mean([a[k] for a in acc]) for k in acc[0]}
         say(E.MODE_NAMES[mode], avg[mode])
     a, b = avg[E.MODE_GRID_F64], avg[E.MODE_GRID_MIXED]
-    for k in ("polarisation", "mean_nn", "clusters"):
+    for k in ("mean_nn", "clusters"):
         assert abs(a[k] - b[k]) <= 0.01 * abs(a[k]), (k, a[k], b[k])
+    floor = 1.0 / np.sqrt(n)
+    assert a["polarisation"] < 3 * floor and b["polarisation"] < 3 * floor
+    assert abs(a["polarisation"] - b["polarisation"]) <= 3 * 0.39 * floor
+
+
+# Strong alignment (radius 0.08, no view cone, weight 0.3) with a narrow
+# speed range orders a flock within ~1000 updates (polarisation ~0.93).
+ORDERED = dict(sep=(0.02, 0.75, 0.002), ali=(0.08, 1.0, 0.3), coh=(0.08, 0.75, 0.008), vmax=0.005, vmin=0.004)
+
+
+def test_long_run_polarisation_ordered_regime():
+    # Polarisation, mean nearest-neighbour distance and cluster count of
+    # FP64 and mixed runs agree within 1% once the flock has ordered:
+    # N = 65536 at density 16384, 3000 updates, averaged over the last 1500
+    # (every 50) and over six seeds, each seed run in both precisions.
+    # (Single chaotic runs differ by up to ~2% in cluster count.)
+    n = 65536
+    F = (n / 16384.0) ** (1.0 / 3.0)
+    sums = {E.MODE_GRID_F64: {}, E.MODE_GRID_MIXED: {}}
+    for seed in range(7, 13):
+        st = B.generate_init(E.INIT_ISOTROPIC, n, F, ORDERED["vmax"], ORDERED["vmin"], seed=seed)
+        for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
+            e = make(n, F, ORDERED, mode)
+            e.set_state(xyzv6=st)
+            e.step(1500)
+            for _ in range(30):
+                e.step(50)
+                for k, v in e.metrics(0.02, 2).items():
+                    sums[mode][k] = sums[mode].get(k, 0.0) + v
+            e.close()
+    a, b = sums[E.MODE_GRID_F64], sums[E.MODE_GRID_MIXED]
+    say("fp64", {k: v / 180 for k, v in a.items()}, "mixed", {k: v / 180 for k, v in b.items()})
+    assert a["polarisation"] / 180 > 0.8 and b["polarisation"] / 180 > 0.8
+    for k in ("polarisation", "mean_nn", "clusters"):
+        assert abs(a[k] - b[k]) <= 0.01 * abs(a[k]), (k, a[k] / 180, b[k] / 180)

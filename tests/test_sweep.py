@@ -129,3 +129,23 @@ def test_create_batch_rejects_bad_tables():
     import ctypes
     h = ctypes.c_void_p()
     assert E.lib().bs_create_batch(None, 0, ctypes.byref(h)) == -1
+
+
+def test_autotune_candidates_and_cache_key():
+    from boidssimulatormpi_amd import autotune as T
+    p = E.make_params(1 << 20, 4.0, mode=E.MODE_GRID_MIXED)
+    assert T.auto_cells(p) == 49          # floor(4 / (0.08 * 1.0001))
+    c = list(T.candidates(p))
+    assert len(c) == len(T.BLOCKS) * len(T.TILES) * len(T.FILLS) * len(T.CELL_SCALES)
+    for t, ps in c:
+        q = ps[0]
+        assert (q.block, q.tile, q.fill) == (t["block"], t["tile"], t["fill"])
+        assert q.cells_per_axis == (0 if t["cell_scale"] == 1.0 else int(49 * t["cell_scale"]))
+        assert q.n == p.n and q.field == p.field and q.mode == p.mode
+    # a cell scale that would leave < 3 cells per axis is dropped
+    small = E.make_params(1000, 0.3, mode=E.MODE_GRID_F64)
+    assert T.auto_cells(small) == 3
+    assert all(t["cell_scale"] == 1.0 for t, _ in T.candidates(small))
+    assert T.workload_key(p) == T.workload_key(E.make_params(1 << 20, 4.0, mode=E.MODE_GRID_MIXED))
+    assert T.workload_key(p) != T.workload_key(E.make_params(1 << 20, 4.1, mode=E.MODE_GRID_MIXED))
+    assert T.workload_key([p, small]).startswith("mode3_flocks2_")
