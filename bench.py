@@ -5,8 +5,13 @@ Workload (default): N = 2^20 boids, cubic field F = 4.0 (density 16384 per
 unit volume), uniform positions, isotropic directions with |v| ~ U[0.001,
 0.005] (seed 2), rules: radii 0.02/0.05/0.08 (separation/alignment/
 cohesion), view angle 270 deg, weights 0.002/0.06/0.008, dt = 1, periodic
-wrap.  A "step" is one full update() of all N boids: fused neighbour +
-integrate kernel, then the counting sort that re-bins the new positions.
+wrap, T = 1000 updates.  A "step" is one full update() of all N boids: fused
+neighbour + integrate kernel, then the counting sort that re-bins the new
+positions.  The timed steps start from the initial state (the warmup steps
+run first and the state is then reset): cohesion clusters the flock as the
+run goes on (mean cohesion neighbours 35 at t = 0, ~118 at t = 1000), so a
+step's cost depends on where in the run it is, and the timed window is the
+configured run itself.
 
 Prints ONE JSON line.  `value` is the mixed-precision path's boid-steps/s
 with the state resident in HBM; the FP64 plain figure is reported beside it.
@@ -46,15 +51,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def tuned_params(mode, init, n, field, tune, block=0):
-    """Engine parameters of one mode: the autotuner's choice (timed on this
-    workload's own initial state) or the defaults."""
+def tuned_params(mode, init, n, field, tune, block=0, evolve=300):
+    """Engine parameters of one mode: the autotuner's choice, timed on this
+    workload's state after `evolve` updates (a clustered state typical of
+    the run), or the defaults."""
     p = B.make_params(n, field, **RULES, mode=mode, flags=E.FLAG_TIMING, block=block)
     if not tune:
         return p, None
     from boidssimulatormpi_amd import autotune as T
-    best, ps, timings = T.tune(p, {"xyzv6": init}, steps=6, warmup=2, cache=False,
-                               blocks=(block,) if block else T.BLOCKS)
+    e = B.Engine(p)
+    e.set_state(xyzv6=init)
+    e.step(evolve)
+    pos, vel = e.state_f64()
+    e.close()
+    best, ps, timings = T.tune(p, {"pos": pos, "vel": vel}, steps=6, warmup=2, cache=False,
+                               blocks=(block,) if block else T.BLOCKS, fills=(100,))
     return ps[0], {"best": best, "candidates": len(timings),
                    "best_ms": min(t["ms_per_step"] for t in timings),
                    "worst_ms": max(t["ms_per_step"] for t in timings)}
@@ -65,6 +76,7 @@ def run_mode(mode, init, n, field, steps, warmup, params):
     info = e.info()
     e.set_state(xyzv6=init)
     e.step(warmup)
+    e.set_state(xyzv6=init)          # the timed steps are the run from t = 0
     e.sync()
     e.reset_timing()
     t0 = time.perf_counter()
@@ -169,18 +181,23 @@ def cpu_baseline(init, n, field, budget_s):
 
 
 def traffic_from_profiles(mode_name, n):
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_pair_{mode_name}_n{n}.json")
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_pair_{mode_name}_n{n}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--field", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=2)
@@ -238,8 +255,8 @@ def main():
         "data": "synthetic (isotropic uniform init, seed %d)" % a.seed,
         "config": {"workload": f"grid boid step, N={n}, F={F}, uniform positions, isotropic velocities "
                                f"|v| in [0.001,0.005], radii 0.02/0.05/0.08, view 270 deg, weights "
-                               f"0.002/0.06/0.008, seed {a.seed}",
-                   "n": n, "field": F, "headline_mode": head["mode"], "parallelism": "single GPU"},
+                               f"0.002/0.06/0.008, seed {a.seed}, timed updates t = 0..{a.steps - 1}",
+                   "n": n, "field": F, "timed_from_t": 0, "headline_mode": head["mode"], "parallelism": "single GPU"},
         "fp64_plain": res.get("grid_f64"),
         "mixed": res.get("grid_mixed"),
         "mixed_over_fp64": (res["grid_mixed"]["boid_steps_per_s"] / res["grid_f64"]["boid_steps_per_s"]

@@ -50,8 +50,8 @@ struct bs_engine {
     uint32_t rows = 0;
     uint64_t groups = 0;               // (row, span group) pairs over all flocks
     int max_g = 0, max_span = 0;
-    int nb = 256;                      // neighbour-kernel workgroup size
-    int tile = 9;                      // LDS tile depth (candidates per thread and pass)
+    int nb = 128;                      // neighbour-kernel workgroup size
+    int tile = 12;                     // LDS tile depth (candidates per thread and pass)
     int fill = 100;                    // chunk size target, percent of nb
     uint32_t chunk = 256;              // boids per chunk at most (fill% of nb)
     bool batch = false;
@@ -528,10 +528,10 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
 
     if (is_grid(e->mode)) {
         if (p->block != 0 && p->block != 128 && p->block != 256) { release(e); return fail(nullptr, BS_E_ARG, "block must be 0, 128 or 256"); }
-        e->nb = p->block ? p->block : 256;
+        e->nb = p->block ? p->block : 128;
         if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
-        e->tile = p->tile ? p->tile : 9;
+        e->tile = p->tile ? p->tile : 12;
         e->fill = p->fill ? p->fill : 100;
         e->chunk = std::max(1u, (uint32_t)(e->fill * e->nb / 100));
         uint64_t cells = 0, rows = 0;

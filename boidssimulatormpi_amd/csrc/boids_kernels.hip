@@ -171,10 +171,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
 
 // Block table of the neighbour kernels.  Every (y,z) row of cells of every
 // flock is cut into groups of `span` consecutive cells, and every group
-// into ceil(n_group/chunk) chunks of consecutive sorted boids of near-equal
-// size, so that a chunk lies in one row, spans at most `span` cells and
-// holds at most `chunk` boids.  Entries are (local row, first slot, end
-// slot, flock).  One workgroup.
+// into chunks of `chunk` consecutive sorted boids (the last one shorter),
+// so that a chunk lies in one row, spans at most `span` cells and holds at
+// most `chunk` boids, and every wavefront but a group's last is full.
+// Entries are (local row, first slot, end slot, flock).  One workgroup.
 __global__ __launch_bounds__(kScanThreads) void k_rows(Flocks F, const uint32_t* start, uint4* table,
                                                         uint32_t* nchunks, uint32_t chunk)
 {
@@ -209,13 +209,8 @@ __global__ __launch_bounds__(kScanThreads) void k_rows(Flocks F, const uint32_t*
                 const uint32_t lo = start[c0 + x0];
                 const uint32_t cnt = start[c0 + x1] - lo;
                 if (!cnt) continue;
-                const uint32_t kc = (cnt + chunk - 1) / chunk;
-                const uint32_t csz = (cnt + kc - 1) / kc;
-                for (uint32_t j = 0; j < kc; ++j) {
-                    const uint32_t t0 = lo + j * csz;
-                    const uint32_t t1 = min(lo + cnt, t0 + csz);
-                    table[k++] = make_uint4(rl, t0, t1, f);
-                }
+                for (uint32_t t0 = lo; t0 < lo + cnt; t0 += chunk)
+                    table[k++] = make_uint4(rl, t0, min(lo + cnt, t0 + chunk), f);
             }
         }
         carry += total;

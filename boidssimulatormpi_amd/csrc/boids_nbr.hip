@@ -27,8 +27,9 @@ namespace boids {
 
 template <int NB, int TD> struct NbrCfg {
     static constexpr int kCap = NB * TD;    // staged candidates per pass
-    static constexpr int kQueue = 16;       // per-lane queue depth (tile indices)
+    static constexpr int kQueue = 16;       // per-lane circular queue depth (tile indices)
 };
+constexpr uint32_t kQMask = 15;             // kQueue - 1
 
 // Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
 // sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg.
@@ -224,19 +225,25 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     };
     // Pairs two at a time, the next two fetched while these are evaluated
     // (indices clamped to the queue, the padding entry skipped).
-    auto drain = [&](uint32_t qn) {
+    // Evaluates the oldest min(queued, cap) entries of this lane's circular
+    // queue, two at a time, the next two fetched while these are evaluated
+    // (positions clamped to the entries taken, the padding entry skipped).
+    auto drain = [&](uint32_t& qh, uint32_t qt, uint32_t cap) {
+        const uint32_t qn = min(qt - qh, cap);
         if (qn == 0) return;
         const uint32_t last = qn - 1;
-        Rec c0 = fetch(sq[threadIdx.x], false);
-        Rec c1 = fetch(sq[min(1u, last) * NB + threadIdx.x], 1 > last);
+        auto at = [&](uint32_t j) { return sq[((qh + min(j, last)) & kQMask) * NB + threadIdx.x]; };
+        Rec c0 = fetch(at(0), false);
+        Rec c1 = fetch(at(1), 1 > last);
         for (uint32_t j = 0; j < qn; j += 2) {
-            const Rec n0 = fetch(sq[min(j + 2, last) * NB + threadIdx.x], j + 2 > last);
-            const Rec n1 = fetch(sq[min(j + 3, last) * NB + threadIdx.x], j + 3 > last);
+            const Rec n0 = fetch(at(j + 2), j + 2 > last);
+            const Rec n1 = fetch(at(j + 3), j + 3 > last);
             eval(c0);
             eval(c1);
             c0 = n0;
             c1 = n1;
         }
+        qh += qn;
     };
 
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
@@ -250,22 +257,26 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         selfk_t = selfk - base;                // wraps past the pass when not in it
         const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
         auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
-        uint32_t qn = 0, k = kb;
+        // Hits go to the tail of this lane's circular queue; when any lane
+        // holds more than kQueue - 4, every lane evaluates its oldest
+        // kQueue / 2 entries (or all it has), so most lanes are busy for
+        // the whole partial drain and no queue overflows.
+        uint32_t qh = 0, qt = 0, k = kb;
         for (; k + 4 <= ke; k += 4) {
-            if (__any(qn > (uint32_t)Cfg::kQueue - 4)) { drain(qn); qn = 0; }
+            if (__any(qt - qh > (uint32_t)Cfg::kQueue - 4)) drain(qh, qt, Cfg::kQueue / 2);
             const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
             const uint32_t kk = k - base;
-            sq[qn * NB + threadIdx.x] = (uint16_t)kk; qn += h0;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)kk; qt += h0;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 1); qt += h1;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 2); qt += h2;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 3); qt += h3;
         }
         for (; k < ke; ++k) {
-            if (__any(qn == (uint32_t)Cfg::kQueue)) { drain(qn); qn = 0; }
-            sq[qn * NB + threadIdx.x] = (uint16_t)(k - base);
-            qn += near(k);
+            if (__any(qt - qh == (uint32_t)Cfg::kQueue)) drain(qh, qt, Cfg::kQueue / 2);
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(k - base);
+            qt += near(k);
         }
-        drain(qn);
+        drain(qh, qt, Cfg::kQueue);
     }
     if (!active) return;
     if (nc != 0) { const double k = (double)nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
@@ -294,17 +305,11 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
 __device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, float c_dn, float c_up, bool c_ok,
                                            bool skip, bool& in, bool& out)
 {
+    const bool cone = P.cone[r];
     const bool s_in = s2 < P.s_lo[r];
     const bool s_out = !(s2 < P.s_hi[r]);
-    if (P.cone[r]) {
-        in = s_in && c_dn >= P.c_hi[r] && c_ok;
-        out = s_out || c_up <= P.c_lo[r];
-    } else {
-        in = s_in;
-        out = s_out;
-    }
-    in = in && !skip;
-    out = out || skip;
+    in = s_in & (!cone | ((c_dn >= P.c_hi[r]) & c_ok)) & !skip;
+    out = s_out | (cone & (c_up <= P.c_lo[r])) | skip;
 }
 
 template <int NB, int TD, bool DIGEST>
@@ -369,7 +374,8 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const bool vbad = !(fvv > 1e-30f);
     const float inv_vn = vbad ? 0.f : __builtin_amdgcn_rsqf(fvv);
 
-    double scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
+    double scx = 0, scy = 0, scz = 0, sax = 0, say = 0, saz = 0;
+    float ssx = 0, ssy = 0, ssz = 0;
     int nc = 0, ns = 0, na = 0;
     uint32_t nre = 0;
     uint64_t hc = 0, hs = 0, ha = 0;
@@ -409,13 +415,15 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             ++nre;
         }
         // 0/1 factors on finite terms (integer displacements, velocities);
-        // the separation term is selected first (it is not finite at d = 0)
+        // the separation factor is selected first (1/d is not finite at d = 0).
+        // Cohesion and alignment (up to ~200 terms) sum in FP64; separation
+        // (a few unit vectors) in FP32.
         const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
+        const float ts = in_s ? -inv_d : 0.f;
         nc += in_c;
         scx = fma((double)ix, mc, scx); scy = fma((double)iy, mc, scy); scz = fma((double)iz, mc, scz);
-        const float ts = in_s ? -inv_d : 0.f;
         ns += in_s;
-        ssx += (double)(gx * ts); ssy += (double)(gy * ts); ssz += (double)(gz * ts);
+        ssx = fmaf(gx, ts, ssx); ssy = fmaf(gy, ts, ssy); ssz = fmaf(gz, ts, ssz);
         na += in_a;
         sax = fma((double)r.w.x, ma, sax); say = fma((double)r.w.y, ma, say); saz = fma((double)r.w.z, ma, saz);
         if (DIGEST) {
@@ -423,19 +431,25 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
         }
     };
-    auto drain = [&](uint32_t qn) {
+    // Evaluates the oldest min(queued, cap) entries of this lane's circular
+    // queue, two at a time, the next two fetched while these are evaluated
+    // (positions clamped to the entries taken, the padding entry skipped).
+    auto drain = [&](uint32_t& qh, uint32_t qt, uint32_t cap) {
+        const uint32_t qn = min(qt - qh, cap);
         if (qn == 0) return;
         const uint32_t last = qn - 1;
-        Rec c0 = fetch(sq[threadIdx.x], false);
-        Rec c1 = fetch(sq[min(1u, last) * NB + threadIdx.x], 1 > last);
+        auto at = [&](uint32_t j) { return sq[((qh + min(j, last)) & kQMask) * NB + threadIdx.x]; };
+        Rec c0 = fetch(at(0), false);
+        Rec c1 = fetch(at(1), 1 > last);
         for (uint32_t j = 0; j < qn; j += 2) {
-            const Rec n0 = fetch(sq[min(j + 2, last) * NB + threadIdx.x], j + 2 > last);
-            const Rec n1 = fetch(sq[min(j + 3, last) * NB + threadIdx.x], j + 3 > last);
+            const Rec n0 = fetch(at(j + 2), j + 2 > last);
+            const Rec n1 = fetch(at(j + 3), j + 3 > last);
             eval(c0);
             eval(c1);
             c0 = n0;
             c1 = n1;
         }
+        qh += qn;
     };
 
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
@@ -450,22 +464,26 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         selfk_t = selfk - base;
         const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
         auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
-        uint32_t qn = 0, k = kb;
+        // Hits go to the tail of this lane's circular queue; when any lane
+        // holds more than kQueue - 4, every lane evaluates its oldest
+        // kQueue / 2 entries (or all it has), so most lanes are busy for
+        // the whole partial drain and no queue overflows.
+        uint32_t qh = 0, qt = 0, k = kb;
         for (; k + 4 <= ke; k += 4) {
-            if (__any(qn > (uint32_t)Cfg::kQueue - 4)) { drain(qn); qn = 0; }
+            if (__any(qt - qh > (uint32_t)Cfg::kQueue - 4)) drain(qh, qt, Cfg::kQueue / 2);
             const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
             const uint32_t kk = k - base;
-            sq[qn * NB + threadIdx.x] = (uint16_t)kk; qn += h0;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 1); qn += h1;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 2); qn += h2;
-            sq[qn * NB + threadIdx.x] = (uint16_t)(kk + 3); qn += h3;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)kk; qt += h0;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 1); qt += h1;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 2); qt += h2;
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 3); qt += h3;
         }
         for (; k < ke; ++k) {
-            if (__any(qn == (uint32_t)Cfg::kQueue)) { drain(qn); qn = 0; }
-            sq[qn * NB + threadIdx.x] = (uint16_t)(k - base);
-            qn += near(k);
+            if (__any(qt - qh == (uint32_t)Cfg::kQueue)) drain(qh, qt, Cfg::kQueue / 2);
+            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(k - base);
+            qt += near(k);
         }
-        drain(qn);
+        drain(qh, qt, Cfg::kQueue);
     }
     if (nre) atomicAdd(recheck, nre);
     if (!active) return;
@@ -484,7 +502,8 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
         d[1] = hc; d[2] = hs; d[3] = ha;
     }
-    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, ssx, ssy, ssz, dax, day, daz,
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, (double)ssx, (double)ssy, (double)ssz,
+                dax, day, daz,
                 out, key_out, rank, count, forces);
 }
 
