@@ -23,7 +23,9 @@ the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
 flocks, N log-uniform in [2^10, 2^16], field (N/16384)^(1/3), vmax ~
 U[0.002, 0.01], vmin = vmax/5, radii x U[0.7, 1.3], mixed precision, seed 3,
 run as one batched engine with graph-replayed steps, next to the same
-kernels launched flock by flock (one engine per flock, eager launches).
+kernels launched flock by flock (one engine per flock: flocks one after
+another as separate jobs, eager or graph-replayed, and all flocks
+interleaved on their own streams).
 """
 from __future__ import annotations
 
@@ -51,20 +53,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def tuned_params(mode, init, n, field, tune, block=0, evolve=300):
-    """Engine parameters of one mode: the autotuner's choice, timed on this
-    workload's state after `evolve` updates (a clustered state typical of
-    the run), or the defaults."""
+def mid_run_state(init, n, field, t):
+    """The workload's state after t updates (FP64 grid engine): a clustered
+    state typical of the run, for tuning and for the CPU sample."""
+    e = B.Engine(B.make_params(n, field, **RULES, mode=E.MODE_GRID_F64))
+    e.set_state(xyzv6=init)
+    e.step(t)
+    pos, vel = e.state_f64()
+    e.close()
+    return pos, vel
+
+
+def tuned_params(mode, mid, n, field, tune, block=0):
+    """Engine parameters of one mode: the autotuner's choice, timed on the
+    mid-run state `mid` = (pos, vel), or the defaults."""
     p = B.make_params(n, field, **RULES, mode=mode, flags=E.FLAG_TIMING, block=block)
     if not tune:
         return p, None
     from boidssimulatormpi_amd import autotune as T
-    e = B.Engine(p)
-    e.set_state(xyzv6=init)
-    e.step(evolve)
-    pos, vel = e.state_f64()
-    e.close()
-    best, ps, timings = T.tune(p, {"pos": pos, "vel": vel}, steps=6, warmup=2, cache=False,
+    best, ps, timings = T.tune(p, {"pos": mid[0], "vel": mid[1]}, steps=6, warmup=2, cache=False,
                                blocks=(block,) if block else T.BLOCKS, fills=(100,))
     return ps[0], {"best": best, "candidates": len(timings),
                    "best_ms": min(t["ms_per_step"] for t in timings),
@@ -126,23 +133,31 @@ def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
                       "gpu_ms_per_step": ms_graph / steps, "sane": sane}
     log(f"[bench] sweep batched: {1e3 * wall / steps:.3f} ms/step")
 
-    for label, flags in (("per_flock", 0), ("per_flock_graph", E.FLAG_GRAPH)):
+    # The same kernels launched flock by flock.  per_flock / per_flock_graph
+    # run the flocks one after another, as the reference runs a sweep (one job
+    # per parameter set, tools/gen_jobscript.py:92-111); per_flock_streams
+    # interleaves all flocks' updates on their engines' own streams.
+    for label, flags, interleave in (("per_flock", 0, False), ("per_flock_graph", E.FLAG_GRAPH, False),
+                                     ("per_flock_streams", 0, True)):
         engines = []
         for j, st in zip(jobs, states):
             e = E.Engine(j.params(E.MODE_GRID_MIXED, flags, block=block))
             e.set_state(xyzv6=st)
             engines.append(e)
-        for _ in range(max(1, warmup // 5)):
-            for e in engines:
-                e.step(1)
         for e in engines:
+            e.step(1)
             e.sync()
         t0 = time.perf_counter()
-        for _ in range(base_steps):
+        if interleave:
+            for _ in range(base_steps):
+                for e in engines:
+                    e.step(1)
             for e in engines:
-                e.step(1)
-        for e in engines:
-            e.sync()
+                e.sync()
+        else:
+            for e in engines:
+                e.step(base_steps)
+                e.sync()
         wall = time.perf_counter() - t0
         for e in engines:
             e.close()
@@ -151,16 +166,19 @@ def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
         log(f"[bench] sweep {label}: {1e3 * wall / base_steps:.3f} ms/step")
     out["speedup_vs_per_flock"] = out["batched"]["boid_steps_per_s"] / out["per_flock"]["boid_steps_per_s"]
     out["speedup_vs_per_flock_graph"] = out["batched"]["boid_steps_per_s"] / out["per_flock_graph"]["boid_steps_per_s"]
+    out["speedup_vs_per_flock_streams"] = out["batched"]["boid_steps_per_s"] / out["per_flock_streams"]["boid_steps_per_s"]
     return out
 
 
-def cpu_baseline(init, n, field, budget_s):
+def cpu_baseline(mid, t_mid, n, field, budget_s):
     """Oracle FP64 cell-list step (the reference algorithm restated, OpenMP)
-    on the same workload; steps until ~budget_s of wall time."""
+    on the same workload, from its state at t = t_mid (a step's cost grows as
+    the flock clusters, so the sample starts mid-run); steps until ~budget_s
+    of wall time."""
     from oracle import oracle as O
     P = O.params(field, RULES["sep"], RULES["ali"], RULES["coh"], RULES["vmax"], RULES["vmin"])
-    p = init[:, :3].astype(np.float64).copy()
-    v = init[:, 3:].astype(np.float64).copy()
+    p = np.ascontiguousarray(mid[0], np.float64).copy()
+    v = np.ascontiguousarray(mid[1], np.float64).copy()
     threads = len(os.sched_getaffinity(0))
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -176,7 +194,7 @@ def cpu_baseline(init, n, field, budget_s):
     except Exception:
         pass
     return {"value": n * steps / wall, "unit": "boid-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} full step(s) of the N={n} workload, oracle FP64 cell list, "
+            "sample": f"{steps} full step(s) of the N={n} workload from its t={t_mid} state, oracle FP64 cell list, "
                       f"{threads} OpenMP threads, {wall:.1f} s wall; CPU: {model}"}
 
 
@@ -205,7 +223,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="neighbour-kernel workgroup size (0: automatic)")
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick block/tile per mode with the autotuner (default: engine defaults, which are the "
+                         "autotuner's choice for this workload)")
     ap.add_argument("--sweep-steps", type=int, default=100, help="batched sweep steps (0: skip the sweep)")
     ap.add_argument("--sweep-base-steps", type=int, default=10, help="steps of the per-flock baseline")
     a = ap.parse_args()
@@ -218,10 +238,15 @@ def main():
     n, F = a.n, a.field
     init = B.generate_init(E.INIT_ISOTROPIC, n, F, RULES["vmax"], RULES["vmin"], seed=a.seed)
     by_name = {v: k for k, v in E.MODE_NAMES.items()}
+    t_mid = min(500, a.steps // 2)
+    mid = None
+    if a.tune or not a.no_cpu:
+        log(f"[bench] mid-run state (t={t_mid}) for tuning and the CPU sample")
+        mid = mid_run_state(init, n, F, t_mid)
     res = {}
     for name in a.modes.split(","):
         log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} steps={a.steps}")
-        params, tuning = tuned_params(by_name[name], init, n, F, not a.no_tune, a.block)
+        params, tuning = tuned_params(by_name[name], mid, n, F, a.tune, a.block)
         res[name] = run_mode(by_name[name], init, n, F, a.steps, a.warmup, params)
         res[name]["tuning"] = tuning
         log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
@@ -239,7 +264,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s)")
-        cpu = cpu_baseline(init, n, F, a.cpu_seconds)
+        cpu = cpu_baseline(mid, t_mid, n, F, a.cpu_seconds)
     out = {
         "metric": "boid-steps/sec at N=2^20: FP64 plain vs mixed precision; % HBM roofline",
         "value": head["boid_steps_per_s"],
