@@ -136,6 +136,34 @@ __device__ __forceinline__ uint32_t to_lattice(double x, double scale)
     return (uint32_t)(int64_t)rint(x * scale);   // two's-complement wrap: x = F maps to 0
 }
 
+// Sub-lattice residuals: x * scale - rint(x * scale) lies in [-1/2, 1/2]
+// lattice units; rounded to kSubBits fractional bits it fits a signed
+// 10-bit field ([-256, 256]).  Three fields pack into the lattice record's
+// spare word (x in bits 0-9, y in 10-19, z in 20-29), so a displacement
+// refined to 2^-kSubBits units costs no extra bytes.  It matters where a
+// term is normalised: the separation unit vector e/|e| of a close pair
+// would otherwise carry the 1/2-unit rounding of both ends.
+constexpr int kSubBits = 9;
+
+__device__ __forceinline__ uint32_t sub_lattice(double x, double scale)
+{
+    const double t = x * scale;
+    const int32_t r = (int32_t)rint((t - rint(t)) * (double)(1 << kSubBits));
+    return (uint32_t)r & 0x3FFu;
+}
+
+__device__ __forceinline__ uint32_t pack_sub(double x, double y, double z, double scale)
+{
+    return sub_lattice(x, scale) | (sub_lattice(y, scale) << 10) | (sub_lattice(z, scale) << 20);
+}
+
+// Field a (0 = x, 1 = y, 2 = z) of pack_sub, sign-extended.
+template <int A>
+__device__ __forceinline__ int32_t sub_field(uint32_t w)
+{
+    return (int32_t)(w << (22 - 10 * A)) >> 22;
+}
+
 // Phase-B + outputs shared by the FP64 and mixed neighbour kernels.
 __device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t cbase, uint32_t s, uint32_t id,
                                             double px, double py, double pz, double vx, double vy,

@@ -65,7 +65,7 @@ struct StateF32 {
 };
 
 struct Lattice {               // mixed mode companion of the sorted FP64 state
-    uint4* p;                  // x, y, z lattice coordinates (w unused)
+    uint4* p;                  // x, y, z lattice coordinates; w: packed sub-lattice residuals
     float4* v;                 // FP32 copies of the velocity (w unused)
 };
 

@@ -272,7 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* per
     if (rank == 0) count[c] = 0u;
     if (with_lat) {
         const double scale = F.fl[flock_of(F, f)].P.lat_scale;   // slots stay inside their flock
-        lat.p[f] = make_uint4(to_lattice(x, scale), to_lattice(y, scale), to_lattice(z, scale), 0u);
+        lat.p[f] = make_uint4(to_lattice(x, scale), to_lattice(y, scale), to_lattice(z, scale),
+                              pack_sub(x, y, z, scale));
         lat.v[f] = make_float4((float)vx, (float)vy, (float)vz, 0.f);
     }
     if (rec) {

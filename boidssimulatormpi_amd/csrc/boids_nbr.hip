@@ -25,11 +25,15 @@
 
 namespace boids {
 
+#ifndef BS_QUEUE_DEPTH
+#define BS_QUEUE_DEPTH 32                   // per-lane circular queue depth, a power of two >= 8
+#endif
 template <int NB, int TD> struct NbrCfg {
     static constexpr int kCap = NB * TD;    // staged candidates per pass
-    static constexpr int kQueue = 16;       // per-lane circular queue depth (tile indices)
+    static constexpr int kQueue = BS_QUEUE_DEPTH;   // per-lane circular queue depth (tile indices)
 };
-constexpr uint32_t kQMask = 15;             // kQueue - 1
+constexpr uint32_t kQMask = BS_QUEUE_DEPTH - 1;
+static_assert((BS_QUEUE_DEPTH & (BS_QUEUE_DEPTH - 1)) == 0 && BS_QUEUE_DEPTH >= 8, "queue depth");
 
 // Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
 // sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg.
@@ -366,6 +370,8 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
     }
     const uint2 mc = coarse_of(me.x, me.y, me.z);
+    const int32_t msx = sub_field<0>(me.w), msy = sub_field<1>(me.w), msz = sub_field<2>(me.w);
+    constexpr float kSub = 1.0f / (float)(1 << kSubBits);
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const v2s my_z = __builtin_bit_cast(v2s, mc.y);
     const double vn = norm3(vx, vy, vz);
@@ -416,14 +422,23 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         }
         // 0/1 factors on finite terms (integer displacements, velocities);
         // the separation factor is selected first (1/d is not finite at d = 0).
-        // Cohesion and alignment (up to ~200 terms) sum in FP64; separation
-        // (a few unit vectors) in FP32.
+        // Cohesion and alignment (up to ~200 terms) sum in FP64 (cohesion on
+        // the exact integer displacements: the lattice's 1/2-unit rounding
+        // averages out of a mean).  Separation sums unit vectors e/|e| in
+        // FP32, on the displacement refined by the sub-lattice residuals:
+        // a close pair's direction would otherwise carry the 1/2-unit
+        // rounding of both ends, ~1e-7 per term at the densest clusters.
+        const uint32_t w = r.q.w;
+        const float ex = fmaf((float)(sub_field<0>(w) - msx), kSub, gx);
+        const float ey = fmaf((float)(sub_field<1>(w) - msy), kSub, gy);
+        const float ez = fmaf((float)(sub_field<2>(w) - msz), kSub, gz);
+        const float e2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
         const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
-        const float ts = in_s ? -inv_d : 0.f;
+        const float ts = in_s ? -__builtin_amdgcn_rsqf(e2) : 0.f;
         nc += in_c;
         scx = fma((double)ix, mc, scx); scy = fma((double)iy, mc, scy); scz = fma((double)iz, mc, scz);
         ns += in_s;
-        ssx = fmaf(gx, ts, ssx); ssy = fmaf(gy, ts, ssy); ssz = fmaf(gz, ts, ssz);
+        ssx = fmaf(ex, ts, ssx); ssy = fmaf(ey, ts, ssy); ssz = fmaf(ez, ts, ssz);
         na += in_a;
         sax = fma((double)r.w.x, ma, sax); say = fma((double)r.w.y, ma, say); saz = fma((double)r.w.z, ma, saz);
         if (DIGEST) {

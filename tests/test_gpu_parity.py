@@ -213,6 +213,66 @@ def test_cfg2_full_size_one_step():
         say(f"{E.MODE_NAMES[mode]} ok, rechecks={e.rechecks()}")
 
 
+def test_clustered_states_neighbour_sets_multi_pass_tiles():
+    # Dense clusters: the headline workload after 300 updates (cells up to
+    # ~60 boids) and the 64-blob clustered init at the 2^24-boid
+    # configuration's density.  Tiles then exceed the LDS capacity and take
+    # several passes (more with tile = 6); neighbour sets must stay exact.
+    n, F = 1 << 20, 4.0
+    e = make(n, F, CFG0, E.MODE_GRID_F64)
+    e.set_state(xyzv6=iso_state(n, F, 2))
+    e.step(300)
+    evolved = e.state_f64()
+    e.close()
+    Fc = (n / (2.0 ** 24 / 1000.0)) ** (1.0 / 3.0)
+    blobs = B.generate_init(E.INIT_CLUSTERED, n, Fc, CFG0["vmax"], CFG0["vmin"], seed=4)
+    for label, Fs, (p0, v0) in (("evolved", F, evolved),
+                                ("blobs", Fc, (blobs[:, :3].astype(np.float64), blobs[:, 3:].astype(np.float64)))):
+        P = O.params(Fs, **CFG0)
+        p, v = p0.copy(), v0.copy()
+        say(f"oracle step ({label}) ...")
+        _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+        for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
+            for tile in (6, 12):
+                g = B.Engine(B.make_params(n, Fs, **CFG0, mode=mode, flags=E.FLAG_DIGEST, tile=tile))
+                g.set_state(pos=p0, vel=v0)
+                g.step(1)
+                bad = np.nonzero((g.digest() != d).any(1))[0]
+                assert bad.size == 0, (label, E.MODE_NAMES[mode], tile, bad.size)
+                gp, gv = g.state_f64()
+                ep, ev = scaled_err(gp, gv, p, v, Fs, CFG0["vmax"])
+                tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+                assert ep <= tol and ev <= tol, (label, E.MODE_NAMES[mode], tile, ep, ev)
+                say(f"{label} {E.MODE_NAMES[mode]} tile {tile} ok, rechecks {g.rechecks()}")
+                g.close()
+
+
+def test_cfg4_full_size_mixed_equals_fp64_neighbour_sets():
+    # N = 2^24, F = 10, 64 Gaussian blobs (seed 4): the oracle would take
+    # minutes here, so the size-independent check is that the mixed path's
+    # neighbour sets equal the FP64 path's (both are exact) for 3 updates
+    # from the same state, and that the state stays in the field with
+    # vmin <= |v| <= vmax.
+    n, F = 1 << 24, 10.0
+    st = B.generate_init(E.INIT_CLUSTERED, n, F, CFG0["vmax"], CFG0["vmin"], seed=4)
+    a = make(n, F, CFG0, E.MODE_GRID_F64, E.FLAG_DIGEST)
+    b = make(n, F, CFG0, E.MODE_GRID_MIXED, E.FLAG_DIGEST)
+    a.set_state(xyzv6=st)
+    b.set_state(xyzv6=st)
+    for t in range(3):
+        a.step(1)
+        b.step(1)
+        da, db = a.digest(), b.digest()
+        bad = np.nonzero((da != db).any(1))[0]
+        assert bad.size == 0, (t, bad.size)
+        pa, va = a.state_f64()
+        b.set_state(pos=pa, vel=va)          # next step from the same state
+    s = np.linalg.norm(va, axis=1)
+    assert np.isfinite(pa).all() and (pa >= 0).all() and (pa <= F).all()
+    assert (s >= CFG0["vmin"] * (1 - 1e-12)).all() and (s <= CFG0["vmax"] * (1 + 1e-12)).all()
+    say(f"cfg4 ok, mixed rechecks {b.rechecks()}")
+
+
 # ---------------------------------------------------------- edge cases ---
 @pytest.mark.parametrize("mode", [E.MODE_ALLPAIRS_F32REF, E.MODE_ALLPAIRS_F64, E.MODE_GRID_F64, E.MODE_GRID_MIXED])
 def test_tiny_and_degenerate_inputs(mode):
