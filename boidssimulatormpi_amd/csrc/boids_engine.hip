@@ -305,6 +305,11 @@ static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
     P->vmin = p->vmin;
     P->vmax_f = (float)p->vmax;
     P->vmin_f = (float)p->vmin;
+    // vmax in [2^(ex-1), 2^ex): quantum 2^(ex-31) <= vmax * 2^-30
+    int ex = -9;
+    if (isfinite(p->vmax) && p->vmax > 0.0) frexp(p->vmax, &ex);
+    P->vq = ldexp(1.0, ex - 31);
+    P->inv_vq = ldexp(1.0, 31 - ex);
     return BS_OK;
 }
 
@@ -531,7 +536,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         e->nb = p->block ? p->block : 128;
         if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
-        e->tile = p->tile ? p->tile : 12;
+        e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? 9 : 12);
         e->fill = p->fill ? p->fill : 100;
         e->chunk = std::max(1u, (uint32_t)(e->fill * e->nb / 100));
         uint64_t cells = 0, rows = 0;

@@ -53,6 +53,10 @@ struct DevParams {
     float c_top;               // c32 >= c_top: too close to 1, recheck
     float band_k;              // k: cosine band growth with 1/d (lattice units)
     int32_t coarse_thr;        // squared distance bound on the top 16 lattice bits
+    // velocity quantum of the mixed lattice records: a power of two with
+    // |v| / vq < 2^31 for |v| <= vmax, so quantised velocities are integers
+    // and their FP64 sums are exact in any order
+    double vq, inv_vq;
 };
 
 struct StateF64 {              // SoA, sorted (grid) or id order (all-pairs)
@@ -66,7 +70,7 @@ struct StateF32 {
 
 struct Lattice {               // mixed mode companion of the sorted FP64 state
     uint4* p;                  // x, y, z lattice coordinates; w: packed sub-lattice residuals
-    float4* v;                 // FP32 copies of the velocity (w unused)
+    float4* v;                 // velocity in quanta of DevParams::vq, integer-valued FP32 (w unused)
 };
 
 // One flock of a grid engine.  A grid engine simulates one or more

@@ -271,10 +271,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* per
     key_sorted[f] = c;
     if (rank == 0) count[c] = 0u;
     if (with_lat) {
-        const double scale = F.fl[flock_of(F, f)].P.lat_scale;   // slots stay inside their flock
+        const DevParams& P = F.fl[flock_of(F, f)].P;   // slots stay inside their flock
+        const double scale = P.lat_scale;
         lat.p[f] = make_uint4(to_lattice(x, scale), to_lattice(y, scale), to_lattice(z, scale),
                               pack_sub(x, y, z, scale));
-        lat.v[f] = make_float4((float)vx, (float)vy, (float)vz, 0.f);
+        const double iq = P.inv_vq;
+        lat.v[f] = make_float4((float)rint(vx * iq), (float)rint(vy * iq), (float)rint(vz * iq), 0.f);
     }
     if (rec) {
         rec[3 * (size_t)f] = make_double2(x, y);

@@ -93,12 +93,11 @@ __device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, v2s my_z)
     return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dzz, dzz, 0, true), true);
 }
 
-// Stages tile positions [base, base + here) (coarse position + slot); each
-// thread copies one run of consecutive positions.  POS(slot) returns the
-// candidate's coarse position.
-template <int NB, typename POS>
+// Stages tile positions [base, base + here); each thread copies one run of
+// consecutive positions, calling STORE(position - base, slot, entry).
+template <int NB, typename STORE>
 __device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* gst, int E, uint32_t base,
-                                           uint32_t here, uint2* tco, uint32_t* tsl, POS pos)
+                                           uint32_t here, STORE store)
 {
     const uint32_t per = (here + NB - 1) / NB;
     const uint32_t p0 = base + threadIdx.x * per;
@@ -113,9 +112,7 @@ __device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* 
     uint32_t eb = off[e], ee = off[e + 1], g = gst[e];
     for (uint32_t p = p0; p < p1; ++p) {
         while (p >= ee) { ++e; eb = ee; ee = off[e + 1]; g = gst[e]; }
-        const uint32_t slot = g + (p - eb);
-        tco[p - base] = pos(slot);
-        tsl[p - base] = slot;
+        store(p - base, g + (p - eb), (uint32_t)e);
     }
 }
 
@@ -253,8 +250,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
         const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
         __syncthreads();                       // previous readers of the tile are done
-        stage_tile<NB>(off, gst, E, base, here, tco, tsl, [&](uint32_t slot) {
-            return coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
+        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t) {
+            tco[kk] = coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
+            tsl[kk] = slot;
         });
         __syncthreads();
         if (!active) continue;
@@ -316,6 +314,26 @@ __device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, 
     out = s_out | (cone & (c_up <= P.c_lo[r])) | skip;
 }
 
+// Lanes of this wave whose bit is set in m below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Per-wave hit list of the mixed kernel, entries (tile entry << 22) |
+// (tile index << 6) | owner lane, appended by ballot and drained in full
+// rounds of 64 (two rounds per step) once it holds kListDrain entries.  The
+// list has 64 more slots where lanes without a hit write (one per lane), so
+// appends need no branches.
+#ifndef BS_LIST_DRAIN
+#define BS_LIST_DRAIN 128
+#endif
+constexpr uint32_t kListDrain = BS_LIST_DRAIN;
+constexpr uint32_t kListCap = kListDrain + 4 * 64;   // + one filter step (4 candidates per lane)
+static_assert(kListDrain % 128 == 0 && kListDrain >= 128, "list drain threshold");
+constexpr float kSepQ = 1073741824.0f;                // separation terms in quanta of 2^-30
+constexpr double kSepQInv = 1.0 / 1073741824.0;
+
 template <int NB, int TD, bool DIGEST>
 __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, StateF64 in, StateF64 out,
                                                       Lattice lat,
@@ -327,9 +345,16 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
                                                       double* forces, uint64_t* digest, uint32_t* recheck)
 {
     using Cfg = NbrCfg<NB, TD>;
-    __shared__ uint2 tco[Cfg::kCap];
-    __shared__ uint32_t tsl[Cfg::kCap];
-    __shared__ uint16_t sq[Cfg::kQueue * NB];
+    constexpr int NW = NB / 64;
+    // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
+    // + 4: a finished lane's look-ahead reads
+    __shared__ uint2 tco[Cfg::kCap + 4];
+    __shared__ uint32_t wl[NW][kListCap + 64];
+    __shared__ uint4 own_p[NB];                 // owner lattice position (w: sub-lattice residuals)
+    __shared__ float4 own_v[NB];                // owner FP32 velocity, 1/|v| (0: unusable)
+    __shared__ double acc[9][NB];               // integer-valued sums: cohesion, separation, alignment
+    __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
+    __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
 
@@ -351,175 +376,212 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     uint32_t* off = (uint32_t*)dyn;
     uint32_t* gst = off + (E + 1);
 
-    const uint32_t s = t0 + threadIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wbase = tid - lane;
+    const uint32_t s = t0 + tid;
     const bool active = s < t1;
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t myid = 0, mystart = 0;
+    uint32_t myid = 0;
     uint4 me = make_uint4(0, 0, 0, 0);
-    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
     int L = 1;
     if (active) {
         px = in.x[s]; py = in.y[s]; pz = in.z[s];
         vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
         myid = in.id[s];
         me = lat.p[s];
-        const float4 fv = lat.v[s];
-        fvx = fv.x; fvy = fv.y; fvz = fv.z;
-        const uint32_t mykey = key_sorted[s];
-        mystart = start[mykey];
-        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
+        L = (int)((key_sorted[s] - cbase) % (uint32_t)G) - cxa + 1;
     }
     const uint2 mc = coarse_of(me.x, me.y, me.z);
-    const int32_t msx = sub_field<0>(me.w), msy = sub_field<1>(me.w), msz = sub_field<2>(me.w);
-    constexpr float kSub = 1.0f / (float)(1 << kSubBits);
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const v2s my_z = __builtin_bit_cast(v2s, mc.y);
-    const double vn = norm3(vx, vy, vz);
-    const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
-    // a vanishing FP32 velocity makes the FP32 cosine meaningless: recheck
-    const bool vbad = !(fvv > 1e-30f);
-    const float inv_vn = vbad ? 0.f : __builtin_amdgcn_rsqf(fvv);
-
-    double scx = 0, scy = 0, scz = 0, sax = 0, say = 0, saz = 0;
-    float ssx = 0, ssy = 0, ssz = 0;
-    int nc = 0, ns = 0, na = 0;
+    {
+        const float fvx = (float)vx, fvy = (float)vy, fvz = (float)vz;
+        const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
+        // a vanishing FP32 velocity makes the FP32 cosine meaningless:
+        // 1/|v| = 0 marks it, and every pair of the owner is rechecked
+        own_p[tid] = me;
+        own_v[tid] = make_float4(fvx, fvy, fvz, (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
+    acc_n[tid] = 0;
+    if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     uint32_t nre = 0;
-    uint64_t hc = 0, hs = 0, ha = 0;
 
     build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
-    const uint32_t selfk = off[9 * L + 4] + (s - mystart);
-    uint32_t selfk_t = 0xFFFFFFFFu;            // own tile index in the current pass
+    uint32_t* list = wl[tid >> 6];
+    constexpr float kSub = 1.0f / (float)(1 << kSubBits);
 
-    struct Rec { uint4 q; float4 w; uint32_t j; bool skip; };
-    auto fetch = [&](uint32_t kk, bool pad) {
+    // Adds the pair (owner o at lattice m, candidate record q, w, slot j)
+    // to the owner's sums.  Every term is an integer-valued double, so the
+    // LDS sums are exact and independent of the order pairs arrive in:
+    // cohesion adds lattice displacements, alignment quantised velocities
+    // (lattice record), separation the FP32 unit vector e/|e| in quanta of
+    // 2^-30, on the displacement refined by the sub-lattice residuals (a
+    // close pair's direction would otherwise carry the 1/2-unit rounding
+    // of both ends).
+    auto accumulate = [&](uint32_t o, const uint4& m, const uint4& q, const float4& w, uint32_t j, bool in_c,
+                          bool in_s, bool in_a) {
+        const int32_t ix = (int32_t)(q.x - m.x), iy = (int32_t)(q.y - m.y), iz = (int32_t)(q.z - m.z);
+        if (in_c) {
+            atomicAdd(&acc[0][o], (double)ix); atomicAdd(&acc[1][o], (double)iy); atomicAdd(&acc[2][o], (double)iz);
+        }
+        if (in_s) {
+            const float ex = fmaf((float)(sub_field<0>(q.w) - sub_field<0>(m.w)), kSub, (float)ix);
+            const float ey = fmaf((float)(sub_field<1>(q.w) - sub_field<1>(m.w)), kSub, (float)iy);
+            const float ez = fmaf((float)(sub_field<2>(q.w) - sub_field<2>(m.w)), kSub, (float)iz);
+            const float t = -__builtin_amdgcn_rsqf(fmaf(ez, ez, fmaf(ey, ey, ex * ex))) * kSepQ;
+            atomicAdd(&acc[3][o], (double)rintf(ex * t));
+            atomicAdd(&acc[4][o], (double)rintf(ey * t));
+            atomicAdd(&acc[5][o], (double)rintf(ez * t));
+        }
+        if (in_a) {
+            atomicAdd(&acc[6][o], (double)w.x); atomicAdd(&acc[7][o], (double)w.y); atomicAdd(&acc[8][o], (double)w.z);
+        }
+        if (in_c | in_s | in_a)
+            atomicAdd(&acc_n[o], (uint64_t)in_c | ((uint64_t)in_s << 21) | ((uint64_t)in_a << 42));
+        if (DIGEST) {
+            const uint64_t h = mix64(in.id[j]);
+            if (in_c) atomicAdd(&acc_h[o], h);
+            if (in_s) atomicAdd(&acc_h[NB + o], h);
+            if (in_a) atomicAdd(&acc_h[2 * NB + o], h);
+        }
+    };
+
+    // One (owner, candidate) pair per lane: the owner's data from LDS, the
+    // candidate's lattice record from L2 by slot; all reads issued ahead
+    // of the evaluation.
+    struct Rec { uint4 q, m; float4 w, ov; uint32_t e, o, j; bool valid; };
+    auto load = [&](const uint32_t* lst, uint32_t idx, uint32_t lim, uint32_t base) {
         Rec r;
-        r.j = tsl[kk];
+        r.valid = idx < lim;
+        r.e = lst[min(idx, lim - 1)];
+        r.o = wbase + (r.e & 63u);
+        const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
+        r.j = gst[e] + (base + ((r.e >> 6) & 0xFFFFu) - off[e]);
         r.q = lat.p[r.j];
         r.w = lat.v[r.j];
-        r.skip = pad || kk == selfk_t;
         return r;
     };
-    auto eval = [&](const Rec& r) {
-        const int32_t ix = (int32_t)(r.q.x - me.x), iy = (int32_t)(r.q.y - me.y), iz = (int32_t)(r.q.z - me.z);
+    // FP32 verdicts outside the guard bands, the oracle's FP64 ones inside.
+    auto eval = [&](Rec r) {
+        r.m = own_p[r.o];
+        r.ov = own_v[r.o];
+        const int32_t ix = (int32_t)(r.q.x - r.m.x), iy = (int32_t)(r.q.y - r.m.y), iz = (int32_t)(r.q.z - r.m.z);
         const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
         const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
         const float inv_d = __builtin_amdgcn_rsqf(s2);
-        const float c = fmaf(fvz, gz, fmaf(fvy, gy, fvx * gx)) * inv_d * inv_vn;
+        const float c = fmaf(r.ov.z, gz, fmaf(r.ov.y, gy, r.ov.x * gx)) * inv_d * r.ov.w;
         const float band = P.band_k * inv_d;
         const float c_dn = c - band, c_up = c + band;
         const bool c_ok = c < P.c_top;
+        const bool skip = !r.valid || r.j == t0 + r.o;    // padding, or the owner itself
         bool in_c, out_c, in_s, out_s, in_a, out_a;
-        rule_bands(P, COH, s2, c_dn, c_up, c_ok, r.skip, in_c, out_c);
-        rule_bands(P, SEP, s2, c_dn, c_up, c_ok, r.skip, in_s, out_s);
-        rule_bands(P, ALI, s2, c_dn, c_up, c_ok, r.skip, in_a, out_a);
+        rule_bands(P, COH, s2, c_dn, c_up, c_ok, skip, in_c, out_c);
+        rule_bands(P, SEP, s2, c_dn, c_up, c_ok, skip, in_s, out_s);
+        rule_bands(P, ALI, s2, c_dn, c_up, c_ok, skip, in_a, out_a);
+        const bool obad = !(r.ov.w > 0.f);
         const bool sure = (in_c || out_c) && (in_s || out_s) && (in_a || out_a) &&
-                          !(vbad && !(out_c && out_s && out_a));
+                          !(obad && !(out_c && out_s && out_a));
         if (!sure) {
-            exact_decide(P, px, py, pz, vx, vy, vz, vn, in.x[r.j], in.y[r.j], in.z[r.j], in_s, in_a, in_c);
+            const uint32_t so = t0 + r.o;
+            const double ovx = in.vx[so], ovy = in.vy[so], ovz = in.vz[so];
+            exact_decide(P, in.x[so], in.y[so], in.z[so], ovx, ovy, ovz, norm3(ovx, ovy, ovz), in.x[r.j], in.y[r.j],
+                         in.z[r.j], in_s, in_a, in_c);
             ++nre;
         }
-        // 0/1 factors on finite terms (integer displacements, velocities);
-        // the separation factor is selected first (1/d is not finite at d = 0).
-        // Cohesion and alignment (up to ~200 terms) sum in FP64 (cohesion on
-        // the exact integer displacements: the lattice's 1/2-unit rounding
-        // averages out of a mean).  Separation sums unit vectors e/|e| in
-        // FP32, on the displacement refined by the sub-lattice residuals:
-        // a close pair's direction would otherwise carry the 1/2-unit
-        // rounding of both ends, ~1e-7 per term at the densest clusters.
-        const uint32_t w = r.q.w;
-        const float ex = fmaf((float)(sub_field<0>(w) - msx), kSub, gx);
-        const float ey = fmaf((float)(sub_field<1>(w) - msy), kSub, gy);
-        const float ez = fmaf((float)(sub_field<2>(w) - msz), kSub, gz);
-        const float e2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-        const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
-        const float ts = in_s ? -__builtin_amdgcn_rsqf(e2) : 0.f;
-        nc += in_c;
-        scx = fma((double)ix, mc, scx); scy = fma((double)iy, mc, scy); scz = fma((double)iz, mc, scz);
-        ns += in_s;
-        ssx = fmaf(ex, ts, ssx); ssy = fmaf(ey, ts, ssy); ssz = fmaf(ez, ts, ssz);
-        na += in_a;
-        sax = fma((double)r.w.x, ma, sax); say = fma((double)r.w.y, ma, say); saz = fma((double)r.w.z, ma, saz);
-        if (DIGEST) {
-            const uint64_t h = mix64(in.id[r.j]);
-            hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
-        }
+        accumulate(r.o, r.m, r.q, r.w, r.j, in_c, in_s, in_a);
     };
-    // Evaluates the oldest min(queued, cap) entries of this lane's circular
-    // queue, two at a time, the next two fetched while these are evaluated
-    // (positions clamped to the entries taken, the padding entry skipped).
-    auto drain = [&](uint32_t& qh, uint32_t qt, uint32_t cap) {
-        const uint32_t qn = min(qt - qh, cap);
-        if (qn == 0) return;
-        const uint32_t last = qn - 1;
-        auto at = [&](uint32_t j) { return sq[((qh + min(j, last)) & kQMask) * NB + threadIdx.x]; };
-        Rec c0 = fetch(at(0), false);
-        Rec c1 = fetch(at(1), 1 > last);
-        for (uint32_t j = 0; j < qn; j += 2) {
-            const Rec n0 = fetch(at(j + 2), j + 2 > last);
-            const Rec n1 = fetch(at(j + 3), j + 3 > last);
+    // Evaluates list[0, lim) in rounds of 64 lanes, two rounds per step,
+    // the next two fetched while these are evaluated.
+    auto drain = [&](uint32_t lim, uint32_t base) {
+        Rec c0 = load(list, lane, lim, base), c1 = load(list, 64 + lane, lim, base);
+        for (uint32_t r0 = 0; r0 < lim; r0 += 128) {
+            const Rec n0 = load(list, r0 + 128 + lane, lim, base), n1 = load(list, r0 + 192 + lane, lim, base);
             eval(c0);
             eval(c1);
             c0 = n0;
             c1 = n1;
         }
-        qh += qn;
     };
 
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
         const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
         __syncthreads();
-        stage_tile<NB>(off, gst, E, base, here, tco, tsl, [&](uint32_t slot) {
+        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
             const uint4 q = lat.p[slot];
-            return coarse_of(q.x, q.y, q.z);
+            const uint2 c = coarse_of(q.x, q.y, q.z);
+            tco[kk] = make_uint2(c.x, c.y | (e << 22));
         });
         __syncthreads();
-        if (!active) continue;
-        selfk_t = selfk - base;
-        const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
-        auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
-        // Hits go to the tail of this lane's circular queue; when any lane
-        // holds more than kQueue - 4, every lane evaluates its oldest
-        // kQueue / 2 entries (or all it has), so most lanes are busy for
-        // the whole partial drain and no queue overflows.
-        uint32_t qh = 0, qt = 0, k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            if (__any(qt - qh > (uint32_t)Cfg::kQueue - 4)) drain(qh, qt, Cfg::kQueue / 2);
-            const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
-            const uint32_t kk = k - base;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)kk; qt += h0;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 1); qt += h1;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 2); qt += h2;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 3); qt += h3;
+        // this lane's window in the staged range [base, base + here) (empty
+        // for idle lanes); positions stay <= base + here, so the look-ahead
+        // reads stay inside tco
+        const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
+        const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
+        // Phase 1: four candidates per lane and step, the next four read
+        // ahead; the wave's hits are appended to its list in (step,
+        // candidate, lane) order.  The owner itself is a hit (coarse
+        // distance 0) and is skipped by the evaluation.
+        uint32_t tail = 0;
+        uint32_t k = kb;
+        uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
+        while (__any(k < ke)) {
+            const uint32_t kn = min(k + 4, ke);
+            const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
+            const uint32_t left = ke - k;
+            auto near = [&](uint2 q) {
+                return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= P.coarse_thr;
+            };
+            const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
+            const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
+            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            const uint32_t n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popcll(m3);
+            const uint32_t e = ((k - base) << 6) | lane;
+            const uint32_t trash = kListCap + lane;
+            list[h0 ? tail + lanes_below(m0) : trash] = (a0.y & 0xFFC00000u) | e;
+            list[h1 ? tail + n0 + lanes_below(m1) : trash] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
+            list[h2 ? tail + n0 + n1 + lanes_below(m2) : trash] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
+            list[h3 ? tail + n0 + n1 + n2 + lanes_below(m3) : trash] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+            tail += n0 + n1 + n2 + n3;
+            if (tail >= kListDrain) {
+                // Phase 2 on every full pair of rounds; the rest moves to the front.
+                const uint32_t lim = tail & ~127u, rem = tail - lim;
+                drain(lim, base);
+                const uint32_t c0 = list[lim + lane], c1 = list[lim + 64 + lane];
+                if (lane < rem) list[lane] = c0;
+                if (lane + 64 < rem) list[64 + lane] = c1;
+                tail = rem;
+            }
+            k = kn;
+            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
         }
-        for (; k < ke; ++k) {
-            if (__any(qt - qh == (uint32_t)Cfg::kQueue)) drain(qh, qt, Cfg::kQueue / 2);
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(k - base);
-            qt += near(k);
-        }
-        drain(qh, qt, Cfg::kQueue);
+        if (tail) drain(tail, base);
     }
     if (nre) atomicAdd(recheck, nre);
     if (!active) return;
+    // this wave's atomics are complete: LDS operations of a wave execute in order
+    const uint64_t cnt = acc_n[tid];
+    const uint32_t nc = (uint32_t)(cnt & 0x1FFFFFu), na = (uint32_t)(cnt >> 42);
     // dv_coh = mean(q) - p = mean(q - p): the lattice sums are already relative
     double dcx = 0, dcy = 0, dcz = 0, dax = 0, day = 0, daz = 0;
     if (nc != 0) {
         const double k = P.unit / (double)nc;
-        dcx = scx * k; dcy = scy * k; dcz = scz * k;
+        dcx = acc[0][tid] * k; dcy = acc[1][tid] * k; dcz = acc[2][tid] * k;
     }
     if (na != 0) {
         const double k = (double)na;
-        dax = sax / k - vx; day = say / k - vy; daz = saz / k - vz;
+        dax = acc[6][tid] * P.vq / k - vx; day = acc[7][tid] * P.vq / k - vy; daz = acc[8][tid] * P.vq / k - vz;
     }
     if (DIGEST) {
         uint64_t* d = digest + 4 * (size_t)myid;
-        d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
-        d[1] = hc; d[2] = hs; d[3] = ha;
+        d[0] = cnt;
+        d[1] = acc_h[tid]; d[2] = acc_h[NB + tid]; d[3] = acc_h[2 * NB + tid];
     }
-    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, (double)ssx, (double)ssy, (double)ssz,
-                dax, day, daz,
-                out, key_out, rank, count, forces);
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, acc[3][tid] * kSepQInv,
+                acc[4][tid] * kSepQInv, acc[5][tid] * kSepQInv, dax, day, daz, out, key_out, rank, count, forces);
 }
 
 // ---------------------------------------------------------------------------

@@ -87,7 +87,7 @@ typedef struct {
     int32_t device;              /* HIP device ordinal */
     int32_t block;               /* grid modes: neighbour-kernel workgroup size, 128 or 256; 0 = automatic (128) */
     int32_t tile;                /* grid modes: LDS tile depth, candidates staged per thread and pass:
-                                  * 6, 9 or 12; 0 = automatic (12) */
+                                  * 6, 9 or 12; 0 = automatic (12 FP64, 9 mixed) */
     int32_t fill;                /* grid modes: most boids per chunk, percent of the workgroup size
                                   * (25..100); 0 = automatic (100) */
 } bs_params;
