@@ -243,7 +243,8 @@ def test_clustered_states_neighbour_sets_multi_pass_tiles():
                 ep, ev = scaled_err(gp, gv, p, v, Fs, CFG0["vmax"])
                 tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
                 assert ep <= tol and ev <= tol, (label, E.MODE_NAMES[mode], tile, ep, ev)
-                say(f"{label} {E.MODE_NAMES[mode]} tile {tile} ok, rechecks {g.rechecks()}")
+                say(f"{label} {E.MODE_NAMES[mode]} tile {tile} ok, rechecks {g.rechecks()}, "
+                    f"max error {ep:.3g} F, {ev:.3g} vmax")
                 g.close()
 
 
