@@ -314,6 +314,16 @@ __device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, 
     out = s_out | (cone & (c_up <= P.c_lo[r])) | skip;
 }
 
+// Value of v in lane src of this wave (all lanes active).
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t src)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+__device__ __forceinline__ float lane_read(float v, uint32_t src)
+{
+    return __builtin_bit_cast(float, lane_read(__builtin_bit_cast(uint32_t, v), src));
+}
+
 // Lanes of this wave whose bit is set in m below the calling lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 {
@@ -350,8 +360,6 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     // + 4: a finished lane's look-ahead reads
     __shared__ uint2 tco[Cfg::kCap + 4];
     __shared__ uint32_t wl[NW][kListCap + 64];
-    __shared__ uint4 own_p[NB];                 // owner lattice position (w: sub-lattice residuals)
-    __shared__ float4 own_v[NB];                // owner FP32 velocity, 1/|v| (0: unusable)
     __shared__ double acc[9][NB];               // integer-valued sums: cohesion, separation, alignment
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
@@ -394,13 +402,15 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const uint2 mc = coarse_of(me.x, me.y, me.z);
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const v2s my_z = __builtin_bit_cast(v2s, mc.y);
+    // Owner data that evaluating lanes read from this lane's registers: the
+    // lattice record and the FP32 velocity with 1/|v|.  A vanishing FP32
+    // velocity makes the FP32 cosine meaningless: 1/|v| = 0 marks it, and
+    // every pair of the owner is rechecked.
+    float4 fv4;
     {
         const float fvx = (float)vx, fvy = (float)vy, fvz = (float)vz;
         const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
-        // a vanishing FP32 velocity makes the FP32 cosine meaningless:
-        // 1/|v| = 0 marks it, and every pair of the owner is rechecked
-        own_p[tid] = me;
-        own_v[tid] = make_float4(fvx, fvy, fvz, (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : 0.f);
+        fv4 = make_float4(fvx, fvy, fvz, (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : 0.f);
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
@@ -467,8 +477,10 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     };
     // FP32 verdicts outside the guard bands, the oracle's FP64 ones inside.
     auto eval = [&](Rec r) {
-        r.m = own_p[r.o];
-        r.ov = own_v[r.o];
+        // every lane of the wave is active here: each reads its owner's registers
+        const uint32_t ol = r.e & 63u;
+        r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), lane_read(me.w, ol));
+        r.ov = make_float4(lane_read(fv4.x, ol), lane_read(fv4.y, ol), lane_read(fv4.z, ol), lane_read(fv4.w, ol));
         const int32_t ix = (int32_t)(r.q.x - r.m.x), iy = (int32_t)(r.q.y - r.m.y), iz = (int32_t)(r.q.z - r.m.z);
         const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
         const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
