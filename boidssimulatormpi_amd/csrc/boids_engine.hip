@@ -50,7 +50,7 @@ struct bs_engine {
     uint32_t rows = 0;
     uint64_t groups = 0;               // (row, span group) pairs over all flocks
     int max_g = 0, max_span = 0;
-    int nb = 128;                      // neighbour-kernel workgroup size
+    int nb = 64;                       // neighbour-kernel workgroup size
     int tile = 12;                     // LDS tile depth (candidates per thread and pass)
     int fill = 100;                    // chunk size target, percent of nb
     uint32_t chunk = 256;              // boids per chunk at most (fill% of nb)
@@ -532,13 +532,14 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     e->P.n = p->n;
 
     if (is_grid(e->mode)) {
-        if (p->block != 0 && p->block != 128 && p->block != 256) { release(e); return fail(nullptr, BS_E_ARG, "block must be 0, 128 or 256"); }
-        e->nb = p->block ? p->block : 128;
+        if (p->block != 0 && p->block != 64 && p->block != 128 && p->block != 256) {
+            release(e);
+            return fail(nullptr, BS_E_ARG, "block must be 0, 64, 128 or 256");
+        }
         if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
-        e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? 9 : 12);
+        e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? 9 : 6);
         e->fill = p->fill ? p->fill : 100;
-        e->chunk = std::max(1u, (uint32_t)(e->fill * e->nb / 100));
         uint64_t cells = 0, rows = 0;
         bool coarse = false;
         for (uint32_t k = 0; k < count; ++k) {
@@ -572,6 +573,12 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             e->flock_field.push_back(q.field);
             e->max_g = std::max(e->max_g, G);
         }
+        // Automatic workgroup size 64: a chunk never leaves its (y,z) row,
+        // so short rows (small flocks) fill 64 lanes better, and at the
+        // smaller LDS footprint more workgroups fit a CU (measured faster
+        // than 128 for long rows too).
+        e->nb = p->block ? p->block : 64;
+        e->chunk = std::max(1u, (uint32_t)(e->fill * e->nb / 100));
         if (e->mode != BS_MODE_ALLPAIRS_F64) {
             if (cells + 1 > 0x7fffffffull) { release(e); return fail(nullptr, BS_E_ARG, "too many grid cells over all flocks"); }
             uint32_t base = 0;

@@ -633,6 +633,9 @@ static void launch_mixed_nb(Flocks F, size_t dyn, dim3 grid, StateF64 in, StateF
 // pairs; false for any other pair.
 #define BS_NBR_DISPATCH(nb, tile, CALL)                                        \
     do {                                                                      \
+        if (nb == 64 && tile == 6) { CALL(64, 6); break; }                    \
+        if (nb == 64 && tile == 9) { CALL(64, 9); break; }                    \
+        if (nb == 64 && tile == 12) { CALL(64, 12); break; }                  \
         if (nb == 128 && tile == 6) { CALL(128, 6); break; }                  \
         if (nb == 128 && tile == 9) { CALL(128, 9); break; }                  \
         if (nb == 128 && tile == 12) { CALL(128, 12); break; }                \

@@ -85,9 +85,9 @@ typedef struct {
     int32_t mode;                /* BS_MODE_* */
     int32_t cells_per_axis;      /* grid modes: 0 = automatic */
     int32_t device;              /* HIP device ordinal */
-    int32_t block;               /* grid modes: neighbour-kernel workgroup size, 128 or 256; 0 = automatic (128) */
+    int32_t block;               /* grid modes: neighbour-kernel workgroup size, 64, 128 or 256; 0 = automatic (64) */
     int32_t tile;                /* grid modes: LDS tile depth, candidates staged per thread and pass:
-                                  * 6, 9 or 12; 0 = automatic (12 FP64, 9 mixed) */
+                                  * 6, 9 or 12; 0 = automatic (6 FP64, 9 mixed) */
     int32_t fill;                /* grid modes: most boids per chunk, percent of the workgroup size
                                   * (25..100); 0 = automatic (100) */
 } bs_params;
