@@ -341,7 +341,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 constexpr uint32_t kListDrain = BS_LIST_DRAIN;
 constexpr uint32_t kListCap = kListDrain + 4 * 64;   // + one filter step (4 candidates per lane)
 static_assert(kListDrain % 128 == 0 && kListDrain >= 128, "list drain threshold");
-constexpr float kSepQ = 1073741824.0f;                // separation terms in quanta of 2^-30
+constexpr double kSepQ = 1073741824.0;                // separation terms in quanta of 2^-30
 constexpr double kSepQInv = 1.0 / 1073741824.0;
 
 template <int NB, int TD, bool DIGEST>
@@ -422,15 +422,15 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
     uint32_t* list = wl[tid >> 6];
-    constexpr float kSub = 1.0f / (float)(1 << kSubBits);
+    constexpr double kSub = 1.0 / (double)(1 << kSubBits);
 
     // Adds the pair (owner o at lattice m, candidate record q, w, slot j)
     // to the owner's sums.  Every term is an integer-valued double, so the
     // LDS sums are exact and independent of the order pairs arrive in:
     // cohesion adds lattice displacements, alignment quantised velocities
-    // (lattice record), separation the FP32 unit vector e/|e| in quanta of
-    // 2^-30, on the displacement refined by the sub-lattice residuals (a
-    // close pair's direction would otherwise carry the 1/2-unit rounding
+    // (lattice record), separation the unit vector e/|e| in quanta of
+    // 2^-30 (FP64), on the displacement refined by the sub-lattice residuals
+    // (a close pair's direction would otherwise carry the 1/2-unit rounding
     // of both ends).
     auto accumulate = [&](uint32_t o, const uint4& m, const uint4& q, const float4& w, uint32_t j, bool in_c,
                           bool in_s, bool in_a) {
@@ -439,13 +439,18 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             atomicAdd(&acc[0][o], (double)ix); atomicAdd(&acc[1][o], (double)iy); atomicAdd(&acc[2][o], (double)iz);
         }
         if (in_s) {
-            const float ex = fmaf((float)(sub_field<0>(q.w) - sub_field<0>(m.w)), kSub, (float)ix);
-            const float ey = fmaf((float)(sub_field<1>(q.w) - sub_field<1>(m.w)), kSub, (float)iy);
-            const float ez = fmaf((float)(sub_field<2>(q.w) - sub_field<2>(m.w)), kSub, (float)iz);
-            const float t = -__builtin_amdgcn_rsqf(fmaf(ez, ez, fmaf(ey, ey, ex * ex))) * kSepQ;
-            atomicAdd(&acc[3][o], (double)rintf(ex * t));
-            atomicAdd(&acc[4][o], (double)rintf(ey * t));
-            atomicAdd(&acc[5][o], (double)rintf(ez * t));
+            // FP64: an FP32 unit vector carries ~1e-7 per term, and dense
+            // clusters sum tens of them
+            const double ex = fma((double)(sub_field<0>(q.w) - sub_field<0>(m.w)), kSub, (double)ix);
+            const double ey = fma((double)(sub_field<1>(q.w) - sub_field<1>(m.w)), kSub, (double)iy);
+            const double ez = fma((double)(sub_field<2>(q.w) - sub_field<2>(m.w)), kSub, (double)iz);
+            const double e2 = fma(ez, ez, fma(ey, ey, ex * ex));
+            double y = __builtin_amdgcn_rsq(e2);           // estimate, then one Newton step
+            y = y * fma(-0.5 * e2, y * y, 1.5);
+            const double t = -y * kSepQ;
+            atomicAdd(&acc[3][o], rint(ex * t));
+            atomicAdd(&acc[4][o], rint(ey * t));
+            atomicAdd(&acc[5][o], rint(ez * t));
         }
         if (in_a) {
             atomicAdd(&acc[6][o], (double)w.x); atomicAdd(&acc[7][o], (double)w.y); atomicAdd(&acc[8][o], (double)w.z);

@@ -213,6 +213,37 @@ def test_cfg2_full_size_one_step():
         say(f"{E.MODE_NAMES[mode]} ok, rechecks={e.rechecks()}")
 
 
+def test_every_workgroup_size_and_tile_depth():
+    # The engine defaults to 64-thread workgroups; 128 and 256 (and every
+    # tile depth) must give the same neighbour sets and states within the
+    # bars, here on a state that cohesion has clustered for 200 updates.
+    n, F = 1 << 16, (2.0 ** 16 / 16384.0) ** (1.0 / 3.0)
+    e = make(n, F, CFG0, E.MODE_GRID_F64)
+    e.set_state(xyzv6=iso_state(n, F, 5))
+    e.step(200)
+    p0, v0 = e.state_f64()
+    e.close()
+    P = O.params(F, **CFG0)
+    p, v = p0.copy(), v0.copy()
+    _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+    for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
+        tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+        for block in (64, 128, 256):
+            for tile in (6, 9, 12):
+                g = B.Engine(B.make_params(n, F, **CFG0, mode=mode, flags=E.FLAG_DIGEST, block=block, tile=tile))
+                info = g.info()
+                assert (info.block, info.tile) == (block, tile)
+                g.set_state(pos=p0, vel=v0)
+                g.step(1)
+                bad = np.nonzero((g.digest() != d).any(1))[0]
+                assert bad.size == 0, (E.MODE_NAMES[mode], block, tile, bad.size)
+                gp, gv = g.state_f64()
+                ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
+                assert ep <= tol and ev <= tol, (E.MODE_NAMES[mode], block, tile, ep, ev)
+                g.close()
+        say(f"{E.MODE_NAMES[mode]}: blocks 64/128/256 x tiles 6/9/12 ok")
+
+
 def test_clustered_states_neighbour_sets_multi_pass_tiles():
     # Dense clusters: the headline workload after 300 updates (cells up to
     # ~60 boids) and the 64-blob clustered init at the 2^24-boid
