@@ -16,7 +16,8 @@ configured run itself.
 Prints ONE JSON line.  `value` is the mixed-precision path's boid-steps/s
 with the state resident in HBM; the FP64 plain figure is reported beside it.
 `roofline` is for the dominant kernel (fused neighbour + integrate) of the
-headline path, from HIP events on the engine's stream; `cpu_baseline` times
+headline path, from HIP events on the engine's stream (`traffic` and `valu`
+are read from the committed rocprofv3 summaries); `cpu_baseline` times
 the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
 
 `sweep` (with --sweep-steps > 0) reports the many-flock parameter sweep: 512
@@ -211,6 +212,24 @@ def traffic_from_profiles(mode_name, n):
     return None
 
 
+def valu_from_profiles(mode_name, n):
+    """VALU issue fraction and active-lane fraction of the fused kernel from
+    the committed rocprofv3 counter summary (uniform-start launches)."""
+    path = os.path.join(ROOT, "profiles", f"valu_pair_kernels_n{n}.txt")
+    if not os.path.exists(path):
+        return None
+    got = {}
+    with open(path) as f:
+        for line in f:
+            for key in ("valu_issue", "active_lanes"):
+                if line.startswith(f"{key}_{mode_name}:"):
+                    got[key] = float(line.split(":", 1)[1])
+    if len(got) != 2:
+        return None
+    return {"issue_frac": got["valu_issue"], "active_lanes": got["active_lanes"],
+            "source": os.path.relpath(path, ROOT), "launches": "first 22 updates (uniform start)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,7 +275,8 @@ def main():
     achieved = BYTES_PER_BOID_STEP * n / (pair["us_avg"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(head["mode"], n),
-                "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": BYTES_PER_BOID_STEP}
+                "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": BYTES_PER_BOID_STEP,
+                "valu": valu_from_profiles(head["mode"], n)}
     sweep = None
     if a.sweep_steps > 0:
         log(f"[bench] sweep: 512 flocks, {a.sweep_steps} batched steps, {a.sweep_base_steps} per-flock steps")

@@ -332,15 +332,20 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 
 // Per-wave hit list of the mixed kernel, entries (tile entry << 22) |
 // (tile index << 6) | owner lane, appended by ballot and drained in full
-// rounds of 64 (two rounds per step) once it holds kListDrain entries.  The
+// rounds of 64 (kRounds rounds per step) once it holds kListDrain entries.  The
 // list has 64 more slots where lanes without a hit write (one per lane), so
 // appends need no branches.
 #ifndef BS_LIST_DRAIN
 #define BS_LIST_DRAIN 128
 #endif
-constexpr uint32_t kListDrain = BS_LIST_DRAIN;
+#ifndef BS_DRAIN_ROUNDS
+#define BS_DRAIN_ROUNDS 4
+#endif
+constexpr uint32_t kRounds = BS_DRAIN_ROUNDS;         // rounds of 64 pairs evaluated per drain step
+constexpr uint32_t kStep = 64 * kRounds;
+constexpr uint32_t kListDrain = BS_LIST_DRAIN > kStep ? BS_LIST_DRAIN : kStep;
 constexpr uint32_t kListCap = kListDrain + 4 * 64;   // + one filter step (4 candidates per lane)
-static_assert(kListDrain % 128 == 0 && kListDrain >= 128, "list drain threshold");
+static_assert(kListDrain % kStep == 0, "list drain threshold");
 constexpr double kSepQ = 1073741824.0;                // separation terms in quanta of 2^-30
 constexpr double kSepQInv = 1.0 / 1073741824.0;
 
@@ -511,16 +516,20 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         }
         accumulate(r.o, r.m, r.q, r.w, r.j, in_c, in_s, in_a);
     };
-    // Evaluates list[0, lim) in rounds of 64 lanes, two rounds per step,
-    // the next two fetched while these are evaluated.
+    // Evaluates list[0, lim) in rounds of 64 lanes, kRounds rounds per
+    // step, the next kRounds fetched while these are evaluated.
     auto drain = [&](uint32_t lim, uint32_t base) {
-        Rec c0 = load(list, lane, lim, base), c1 = load(list, 64 + lane, lim, base);
-        for (uint32_t r0 = 0; r0 < lim; r0 += 128) {
-            const Rec n0 = load(list, r0 + 128 + lane, lim, base), n1 = load(list, r0 + 192 + lane, lim, base);
-            eval(c0);
-            eval(c1);
-            c0 = n0;
-            c1 = n1;
+        Rec c[kRounds];
+#pragma unroll
+        for (uint32_t i = 0; i < kRounds; ++i) c[i] = load(list, 64 * i + lane, lim, base);
+        for (uint32_t r0 = 0; r0 < lim; r0 += kStep) {
+            Rec nx[kRounds];
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) nx[i] = load(list, r0 + kStep + 64 * i + lane, lim, base);
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) eval(c[i]);
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) c[i] = nx[i];
         }
     };
 
@@ -564,12 +573,15 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             list[h3 ? tail + n0 + n1 + n2 + lanes_below(m3) : trash] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
             tail += n0 + n1 + n2 + n3;
             if (tail >= kListDrain) {
-                // Phase 2 on every full pair of rounds; the rest moves to the front.
-                const uint32_t lim = tail & ~127u, rem = tail - lim;
+                // Phase 2 on every full step of rounds; the rest moves to the front.
+                const uint32_t lim = tail - tail % kStep, rem = tail - lim;
                 drain(lim, base);
-                const uint32_t c0 = list[lim + lane], c1 = list[lim + 64 + lane];
-                if (lane < rem) list[lane] = c0;
-                if (lane + 64 < rem) list[64 + lane] = c1;
+                uint32_t mv[kRounds];
+#pragma unroll
+                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + 64 * i + lane];
+#pragma unroll
+                for (uint32_t i = 0; i < kRounds; ++i)
+                    if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
                 tail = rem;
             }
             k = kn;
