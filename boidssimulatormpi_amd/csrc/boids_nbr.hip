@@ -11,29 +11,26 @@
 // one contiguous tile range [off(L-1, 0), off(L+2, 0)).
 //
 // Per candidate the tile holds a coarse position (top 16 bits of the 32-bit
-// lattice coordinate per axis) and the sorted slot.  Phase 1 tests every
-// candidate of the window with a packed 16-bit squared distance that never
-// drops a pair inside the largest radius, and appends hits to a per-lane LDS
-// queue; phase 2 drains the queue densely, fetching the candidates' full
-// records from L2 by slot, and takes the rule decisions:
+// lattice coordinate per axis) and the index of its tile entry (cell); the
+// slot is the entry's sorted start + offset.  Phase 1 tests every candidate
+// of the window with a packed 16-bit squared distance that never drops a
+// pair inside the largest radius, lanes of one cell reading the same tile
+// entry at a time, and appends hits to the wave's list by ballot.  Phase 2
+// evaluates the list 64 pairs at a time, every lane busy, fetching each
+// candidate's record from L2 by slot, takes the rule decisions
 //   FP64 plain: the oracle's expressions (exact thresholds, s < s_thr and
 //               c >= c_thr && c <= 1, on the minimum image);
 //   mixed:      FP32 on exact lattice differences outside guard bands,
-//               the FP64 expression inside them.
+//               the FP64 expression inside them,
+// and adds the pair to its owner's sums with LDS atomics.
 // Tiles larger than the LDS capacity are processed in passes.
 #include "boids_device.h"
 
 namespace boids {
 
-#ifndef BS_QUEUE_DEPTH
-#define BS_QUEUE_DEPTH 32                   // per-lane circular queue depth, a power of two >= 8
-#endif
 template <int NB, int TD> struct NbrCfg {
     static constexpr int kCap = NB * TD;    // staged candidates per pass
-    static constexpr int kQueue = BS_QUEUE_DEPTH;   // per-lane circular queue depth (tile indices)
 };
-constexpr uint32_t kQMask = BS_QUEUE_DEPTH - 1;
-static_assert((BS_QUEUE_DEPTH & (BS_QUEUE_DEPTH - 1)) == 0 && BS_QUEUE_DEPTH >= 8, "queue depth");
 
 // Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
 // sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg.
@@ -116,9 +113,53 @@ __device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* 
     }
 }
 
+// Value of v in lane src of this wave (all lanes active).
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t src)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+__device__ __forceinline__ float lane_read(float v, uint32_t src)
+{
+    return __builtin_bit_cast(float, lane_read(__builtin_bit_cast(uint32_t, v), src));
+}
+
+// Lanes of this wave whose bit is set in m below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Per-wave hit list of the neighbour kernels, entries (tile entry << 22) |
+// (tile index << 6) | owner lane, appended by ballot and drained in full
+// rounds of 64 (kRounds rounds per step) once it holds kDrain entries.  The
+// list has 64 more slots where lanes without a hit write (one per lane), so
+// appends need no branches.
+#ifndef BS_LIST_DRAIN
+#define BS_LIST_DRAIN 128
+#endif
+#ifndef BS_MIXED_ROUNDS
+#define BS_MIXED_ROUNDS 4
+#endif
+#ifndef BS_F64_ROUNDS
+#define BS_F64_ROUNDS 2
+#endif
+template <uint32_t R> struct ListCfg {
+    static constexpr uint32_t kRounds = R;                // rounds of 64 pairs evaluated per drain step
+    static constexpr uint32_t kStep = 64 * R;
+    static constexpr uint32_t kDrain = BS_LIST_DRAIN > kStep ? BS_LIST_DRAIN : kStep;
+    static constexpr uint32_t kCap = kDrain + 4 * 64;     // + one filter step (4 candidates per lane)
+    static_assert(kDrain % kStep == 0, "list drain threshold");
+};
+
 // ---------------------------------------------------------------------------
 // FP64 plain.
 // ---------------------------------------------------------------------------
+// Phase 1 as in the mixed kernel (coarse test, per-wave hit list); phase 2
+// evaluates the list 64 pairs at a time with the oracle's expressions on
+// the minimum image and adds each pair to its owner's sums with LDS
+// atomics.  The sums are FP64 in an order that
+// depends on the list, not the oracle's ascending order (deterministic for
+// a given state; they differ from the oracle's in the last bits only).
 template <int NB, int TD, bool DIGEST>
 __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, StateF64 in,
                                                     const double2* __restrict__ rec, StateF64 out,
@@ -130,9 +171,18 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
                                                     double* forces, uint64_t* digest)
 {
     using Cfg = NbrCfg<NB, TD>;
-    __shared__ uint2 tco[Cfg::kCap];
-    __shared__ uint32_t tsl[Cfg::kCap];
-    __shared__ uint16_t sq[Cfg::kQueue * NB];   // per-lane queues, lane-interleaved
+    using LC = ListCfg<BS_F64_ROUNDS>;
+    constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
+    constexpr int NW = NB / 64;
+    // coarse positions, tile entry in bits 22-31 of .y; + 4: a finished
+    // lane's look-ahead reads (positions stay <= base + here)
+    __shared__ uint2 tco[Cfg::kCap + 4];
+    __shared__ uint32_t wl[NW][kListCap + 64];
+    __shared__ double2 own_a[NB], own_b[NB], own_c[NB];   // owner (x, y), (z, vx), (vy, vz)
+    __shared__ double own_n[NB];                // owner |v|
+    __shared__ double acc[9][NB];               // sums of q (cohesion), e/|e| (separation), v (alignment)
+    __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
+    __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
 
@@ -154,142 +204,164 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     uint32_t* off = (uint32_t*)dyn;
     uint32_t* gst = off + (E + 1);
 
-    const uint32_t s = t0 + threadIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wbase = tid - lane;
+    const uint32_t s = t0 + tid;
     const bool active = s < t1;
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t myid = 0, mystart = 0;
+    uint32_t myid = 0;
     int L = 1;
     if (active) {
         px = in.x[s]; py = in.y[s]; pz = in.z[s];
         vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
         myid = in.id[s];
-        const uint32_t mykey = key_sorted[s];
-        mystart = start[mykey];
-        L = (int)((mykey - cbase) % (uint32_t)G) - cxa + 1;
+        L = (int)((key_sorted[s] - cbase) % (uint32_t)G) - cxa + 1;
     }
     const double sc = P.lat_scale;
     const uint2 mc = coarse_of(to_lattice(px, sc), to_lattice(py, sc), to_lattice(pz, sc));
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const v2s my_z = __builtin_bit_cast(v2s, mc.y);
-    const double vn = norm3(vx, vy, vz);
+    own_a[tid] = make_double2(px, py);
+    own_b[tid] = make_double2(pz, vx);
+    own_c[tid] = make_double2(vy, vz);
+    own_n[tid] = norm3(vx, vy, vz);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
+    acc_n[tid] = 0;
+    if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     const double F = P.F, half = P.half;
     const bool k_c = P.cone[COH], k_s = P.cone[SEP], k_a = P.cone[ALI];
-
-    double scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
-    int nc = 0, ns = 0, na = 0;
-    uint64_t hc = 0, hs = 0, ha = 0;
 
     build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
-    const uint32_t selfk = off[9 * L + 4] + (s - mystart);
-    uint32_t selfk_t = 0xFFFFFFFFu;            // own tile index in the current pass
+    uint32_t* list = wl[tid >> 6];
 
-    // One queued candidate: its record, fetched from L2 by slot; `skip`
-    // marks the boid itself and the padding entry of an odd queue.
-    struct Rec { double x, y, z, vx, vy, vz; uint32_t id; bool skip; };
-    auto fetch = [&](uint32_t kk, bool pad) {
+    // One (owner, candidate) pair per lane: the candidate's record from L2
+    // by slot, read ahead of the evaluation.
+    struct Rec { double2 a, b, c; uint32_t e, o, j; bool valid; };
+    auto load = [&](uint32_t idx, uint32_t lim, uint32_t base) {
         Rec r;
-        const uint32_t j = tsl[kk];
-        const double2 a = rec[3 * (size_t)j], b = rec[3 * (size_t)j + 1], c = rec[3 * (size_t)j + 2];
-        r.x = a.x; r.y = a.y; r.z = b.x;
-        r.vx = b.y; r.vy = c.x; r.vz = c.y;
-        r.id = DIGEST ? in.id[j] : 0u;
-        r.skip = pad || kk == selfk_t;
+        r.valid = idx < lim;
+        r.e = list[min(idx, lim - 1)];
+        r.o = wbase + (r.e & 63u);
+        const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
+        r.j = gst[e] + (base + ((r.e >> 6) & 0xFFFFu) - off[e]);
+        r.a = rec[3 * (size_t)r.j];
+        r.b = rec[3 * (size_t)r.j + 1];
+        r.c = rec[3 * (size_t)r.j + 2];
         return r;
     };
-    // The oracle's decisions on the minimum image; branch-free sums (an
-    // fma with a 0/1 factor adds q or leaves the sum unchanged, exactly as
-    // the oracle's conditional add does) so that consecutive pairs overlap.
+    // The oracle's decisions on the minimum image (src/boid.cpp:27-42,
+    // src/boid_simulation.cpp:153-212), the pair added to its owner's sums.
     auto eval = [&](const Rec& r) {
-        const double qx = min_image(r.x, px, F, half);
-        const double qy = min_image(r.y, py, F, half);
-        const double qz = min_image(r.z, pz, F, half);
-        const double ex = px - qx, ey = py - qy, ez = pz - qz;
+        const uint32_t o = r.o;
+        const double2 oa = own_a[o], ob = own_b[o], oc = own_c[o];
+        const double on = own_n[o];
+        const double ox = oa.x, oy = oa.y, oz = ob.x, ovx = ob.y, ovy = oc.x, ovz = oc.y;
+        const double qx = min_image(r.a.x, ox, F, half);
+        const double qy = min_image(r.a.y, oy, F, half);
+        const double qz = min_image(r.b.x, oz, F, half);
+        const double ex = ox - qx, ey = oy - qy, ez = oz - qz;
         const double s2 = ex * ex + ey * ey + ez * ez;
-        const double gx = qx - px, gy = qy - py, gz = qz - pz;
-        const double dot = vx * gx + vy * gy + vz * gz;
+        const double gx = qx - ox, gy = qy - oy, gz = qz - oz;
+        const double dot = ovx * gx + ovy * gy + ovz * gz;
         const double t = sqrt(s2);
-        const double c = dot / (vn * t);
+        const double c = dot / (on * t);
         const bool cin = c <= 1.0;
-        const bool in_c = !r.skip && s2 < P.s_thr[COH] && (!k_c || (c >= P.c_thr[COH] && cin));
-        const bool in_s = !r.skip && s2 < P.s_thr[SEP] && (!k_s || (c >= P.c_thr[SEP] && cin));
-        const bool in_a = !r.skip && s2 < P.s_thr[ALI] && (!k_a || (c >= P.c_thr[ALI] && cin));
-        const double mc = in_c ? 1.0 : 0.0, ma = in_a ? 1.0 : 0.0;
-        nc += in_c; scx = fma(qx, mc, scx); scy = fma(qy, mc, scy); scz = fma(qz, mc, scz);
-        na += in_a; sax = fma(r.vx, ma, sax); say = fma(r.vy, ma, say); saz = fma(r.vz, ma, saz);
-        if (in_s) { ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t; }
+        const bool skip = !r.valid || r.j == t0 + o;      // padding, or the owner itself
+        const bool in_c = !skip && s2 < P.s_thr[COH] && (!k_c || (c >= P.c_thr[COH] && cin));
+        const bool in_s = !skip && s2 < P.s_thr[SEP] && (!k_s || (c >= P.c_thr[SEP] && cin));
+        const bool in_a = !skip && s2 < P.s_thr[ALI] && (!k_a || (c >= P.c_thr[ALI] && cin));
+        if (in_c) { atomicAdd(&acc[0][o], qx); atomicAdd(&acc[1][o], qy); atomicAdd(&acc[2][o], qz); }
+        if (in_s) { atomicAdd(&acc[3][o], ex / t); atomicAdd(&acc[4][o], ey / t); atomicAdd(&acc[5][o], ez / t); }
+        if (in_a) { atomicAdd(&acc[6][o], r.b.y); atomicAdd(&acc[7][o], r.c.x); atomicAdd(&acc[8][o], r.c.y); }
+        if (in_c | in_s | in_a)
+            atomicAdd(&acc_n[o], (uint64_t)in_c | ((uint64_t)in_s << 21) | ((uint64_t)in_a << 42));
         if (DIGEST) {
-            const uint64_t h = mix64(r.id);
-            hc += in_c ? h : 0; hs += in_s ? h : 0; ha += in_a ? h : 0;
+            const uint64_t h = mix64(in.id[r.j]);
+            if (in_c) atomicAdd(&acc_h[o], h);
+            if (in_s) atomicAdd(&acc_h[NB + o], h);
+            if (in_a) atomicAdd(&acc_h[2 * NB + o], h);
         }
     };
-    // Pairs two at a time, the next two fetched while these are evaluated
-    // (indices clamped to the queue, the padding entry skipped).
-    // Evaluates the oldest min(queued, cap) entries of this lane's circular
-    // queue, two at a time, the next two fetched while these are evaluated
-    // (positions clamped to the entries taken, the padding entry skipped).
-    auto drain = [&](uint32_t& qh, uint32_t qt, uint32_t cap) {
-        const uint32_t qn = min(qt - qh, cap);
-        if (qn == 0) return;
-        const uint32_t last = qn - 1;
-        auto at = [&](uint32_t j) { return sq[((qh + min(j, last)) & kQMask) * NB + threadIdx.x]; };
-        Rec c0 = fetch(at(0), false);
-        Rec c1 = fetch(at(1), 1 > last);
-        for (uint32_t j = 0; j < qn; j += 2) {
-            const Rec n0 = fetch(at(j + 2), j + 2 > last);
-            const Rec n1 = fetch(at(j + 3), j + 3 > last);
-            eval(c0);
-            eval(c1);
-            c0 = n0;
-            c1 = n1;
+    auto drain = [&](uint32_t lim, uint32_t base) {
+        Rec c[kRounds];
+#pragma unroll
+        for (uint32_t i = 0; i < kRounds; ++i) c[i] = load(64 * i + lane, lim, base);
+        for (uint32_t r0 = 0; r0 < lim; r0 += kStep) {
+            Rec nx[kRounds];
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) nx[i] = load(r0 + kStep + 64 * i + lane, lim, base);
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) eval(c[i]);
+#pragma unroll
+            for (uint32_t i = 0; i < kRounds; ++i) c[i] = nx[i];
         }
-        qh += qn;
     };
 
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
         const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
-        __syncthreads();                       // previous readers of the tile are done
-        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t) {
-            tco[kk] = coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
-            tsl[kk] = slot;
+        __syncthreads();
+        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
+            const uint2 c = coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
+            tco[kk] = make_uint2(c.x, c.y | (e << 22));
         });
         __syncthreads();
-        if (!active) continue;
-        selfk_t = selfk - base;                // wraps past the pass when not in it
-        const uint32_t kb = max(wlo, base), ke = min(whi, base + here);
-        auto near = [&](uint32_t k) { return coarse_d2(tco[k - base], my_xy, my_z) <= P.coarse_thr; };
-        // Hits go to the tail of this lane's circular queue; when any lane
-        // holds more than kQueue - 4, every lane evaluates its oldest
-        // kQueue / 2 entries (or all it has), so most lanes are busy for
-        // the whole partial drain and no queue overflows.
-        uint32_t qh = 0, qt = 0, k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            if (__any(qt - qh > (uint32_t)Cfg::kQueue - 4)) drain(qh, qt, Cfg::kQueue / 2);
-            const bool h0 = near(k), h1 = near(k + 1), h2 = near(k + 2), h3 = near(k + 3);
-            const uint32_t kk = k - base;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)kk; qt += h0;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 1); qt += h1;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 2); qt += h2;
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(kk + 3); qt += h3;
+        const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
+        const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
+        uint32_t tail = 0;
+        uint32_t k = kb;
+        uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
+        while (__any(k < ke)) {
+            const uint32_t kn = min(k + 4, ke);
+            const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
+            const uint32_t left = ke - k;
+            auto near = [&](uint2 q) {
+                return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= P.coarse_thr;
+            };
+            const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
+            const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
+            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            const uint32_t n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popcll(m3);
+            const uint32_t e = ((k - base) << 6) | lane;
+            const uint32_t trash = kListCap + lane;
+            list[h0 ? tail + lanes_below(m0) : trash] = (a0.y & 0xFFC00000u) | e;
+            list[h1 ? tail + n0 + lanes_below(m1) : trash] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
+            list[h2 ? tail + n0 + n1 + lanes_below(m2) : trash] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
+            list[h3 ? tail + n0 + n1 + n2 + lanes_below(m3) : trash] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+            tail += n0 + n1 + n2 + n3;
+            if (tail >= kListDrain) {
+                const uint32_t lim = tail - tail % kStep, rem = tail - lim;
+                drain(lim, base);
+                uint32_t mv[kRounds];
+#pragma unroll
+                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + 64 * i + lane];
+#pragma unroll
+                for (uint32_t i = 0; i < kRounds; ++i)
+                    if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
+                tail = rem;
+            }
+            k = kn;
+            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
         }
-        for (; k < ke; ++k) {
-            if (__any(qt - qh == (uint32_t)Cfg::kQueue)) drain(qh, qt, Cfg::kQueue / 2);
-            sq[(qt & kQMask) * NB + threadIdx.x] = (uint16_t)(k - base);
-            qt += near(k);
-        }
-        drain(qh, qt, Cfg::kQueue);
+        if (tail) drain(tail, base);
     }
     if (!active) return;
+    // this wave's atomics are complete: LDS operations of a wave execute in order
+    const uint64_t cnt = acc_n[tid];
+    const uint32_t nc = (uint32_t)(cnt & 0x1FFFFFu), na = (uint32_t)(cnt >> 42);
+    double scx = acc[0][tid], scy = acc[1][tid], scz = acc[2][tid];
+    double sax = acc[6][tid], say = acc[7][tid], saz = acc[8][tid];
     if (nc != 0) { const double k = (double)nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
     if (na != 0) { const double k = (double)na; sax = sax / k - vx; say = say / k - vy; saz = saz / k - vz; }
     if (DIGEST) {
         uint64_t* d = digest + 4 * (size_t)myid;
-        d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
-        d[1] = hc; d[2] = hs; d[3] = ha;
+        d[0] = cnt;
+        d[1] = acc_h[tid]; d[2] = acc_h[NB + tid]; d[3] = acc_h[2 * NB + tid];
     }
-    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, scx, scy, scz, ssx, ssy, ssz, sax, say, saz,
-                out, key_out, rank, count, forces);
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, scx, scy, scz, acc[3][tid], acc[4][tid], acc[5][tid],
+                sax, say, saz, out, key_out, rank, count, forces);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,38 +386,6 @@ __device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, 
     out = s_out | (cone & (c_up <= P.c_lo[r])) | skip;
 }
 
-// Value of v in lane src of this wave (all lanes active).
-__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t src)
-{
-    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
-}
-__device__ __forceinline__ float lane_read(float v, uint32_t src)
-{
-    return __builtin_bit_cast(float, lane_read(__builtin_bit_cast(uint32_t, v), src));
-}
-
-// Lanes of this wave whose bit is set in m below the calling lane.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Per-wave hit list of the mixed kernel, entries (tile entry << 22) |
-// (tile index << 6) | owner lane, appended by ballot and drained in full
-// rounds of 64 (kRounds rounds per step) once it holds kListDrain entries.  The
-// list has 64 more slots where lanes without a hit write (one per lane), so
-// appends need no branches.
-#ifndef BS_LIST_DRAIN
-#define BS_LIST_DRAIN 128
-#endif
-#ifndef BS_DRAIN_ROUNDS
-#define BS_DRAIN_ROUNDS 4
-#endif
-constexpr uint32_t kRounds = BS_DRAIN_ROUNDS;         // rounds of 64 pairs evaluated per drain step
-constexpr uint32_t kStep = 64 * kRounds;
-constexpr uint32_t kListDrain = BS_LIST_DRAIN > kStep ? BS_LIST_DRAIN : kStep;
-constexpr uint32_t kListCap = kListDrain + 4 * 64;   // + one filter step (4 candidates per lane)
-static_assert(kListDrain % kStep == 0, "list drain threshold");
 constexpr double kSepQ = 1073741824.0;                // separation terms in quanta of 2^-30
 constexpr double kSepQInv = 1.0 / 1073741824.0;
 
@@ -360,6 +400,8 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
                                                       double* forces, uint64_t* digest, uint32_t* recheck)
 {
     using Cfg = NbrCfg<NB, TD>;
+    using LC = ListCfg<BS_MIXED_ROUNDS>;
+    constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
     constexpr int NW = NB / 64;
     // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
     // + 4: a finished lane's look-ahead reads
