@@ -20,6 +20,11 @@ headline path, from HIP events on the engine's stream (`traffic` and `valu`
 are read from the committed rocprofv3 summaries); `cpu_baseline` times
 the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
 
+`--workload flock16k` (N=16384, F=1, T=8000; the CPU beside it is the
+reference's own grid engine built from source, oracle/_ref/ref_grid) and
+`--workload blobs16m` (N=2^24, F=10, 64 clustered blobs, T=500, mixed)
+measure the same path at the other configured sizes.
+
 `sweep` (with --sweep-steps > 0) reports the many-flock parameter sweep: 512
 flocks, N log-uniform in [2^10, 2^16], field (N/16384)^(1/3), vmax ~
 U[0.002, 0.01], vmin = vmax/5, radii x U[0.7, 1.3], mixed precision, seed 3,
@@ -46,6 +51,19 @@ import boidssimulatormpi_amd as B  # noqa: E402
 from boidssimulatormpi_amd import engine as E  # noqa: E402
 
 RULES = dict(sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008), vmax=0.005, vmin=0.001)
+# Workloads (all with RULES): the headline (default) and two more shapes of
+# the same path, each run from t = 0 for its configured number of updates.
+WORKLOADS = {
+    "flock1m": dict(n=1 << 20, field=4.0, seed=2, init=E.INIT_ISOTROPIC, steps=1000,
+                    modes="grid_f64,grid_mixed", head="grid_mixed", cpu="port", sweep=True,
+                    what="uniform positions, isotropic velocities"),
+    "flock16k": dict(n=16384, field=1.0, seed=1, init=E.INIT_ISOTROPIC, steps=8000,
+                     modes="grid_f64,grid_mixed", head="grid_f64", cpu="reference", sweep=False,
+                     what="uniform positions, isotropic velocities"),
+    "blobs16m": dict(n=1 << 24, field=10.0, seed=4, init=E.INIT_CLUSTERED, steps=500,
+                     modes="grid_mixed", head="grid_mixed", cpu=None, sweep=False,
+                     what="64 Gaussian blobs (sigma 0.05 F), per-blob headings"),
+}
 BYTES_PER_BOID_STEP = 96          # read + write of the FP64 master state (pos + vel, 6 x 8 B)
 HBM_PEAK_GBS = 8000.0
 
@@ -171,6 +189,42 @@ def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
     return out
 
 
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        return next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        return ""
+
+
+def cpu_reference_grid(n, field, seed, budget_s):
+    """The reference's own grid engine (src/boid_simulation_grid.cpp, compiled
+    from source into oracle/_ref/ref_grid by oracle/Makefile.ref) on the same
+    shape of workload (its random_uniform init): per-update time from two runs
+    of different length, so setup and init cancel.  None when not built."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_grid")
+    if not os.path.exists(exe):
+        return None
+    threads = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = RULES
+    def run(t):
+        args = [exe, os.devnull, str(n), str(t), repr(field), *(repr(x) for x in (*r["sep"], *r["ali"], *r["coh"])),
+                repr(r["vmax"]), repr(r["vmin"]), "random_uniform", str(seed), ""]
+        t0 = time.perf_counter()
+        subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL, timeout=600)
+        return time.perf_counter() - t0
+    w1 = run(1)
+    per = max(run(2) - w1, 1e-3)                         # one update, for sizing
+    k = max(2, int(budget_s / per))
+    wk = run(1 + k)
+    per = (wk - w1) / k
+    return {"value": n / per, "unit": "boid-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"{k} updates of the reference grid engine (oracle/_ref/ref_grid, src/boid_simulation_grid.cpp "
+                      f"built from source), N={n}, F={field}, random_uniform init seed {seed}, {threads} OpenMP "
+                      f"threads: (time of 1+{k} updates - time of 1) / {k}; CPU: {cpu_model()}"}
+
+
 def cpu_baseline(mid, t_mid, n, field, budget_s):
     """Oracle FP64 cell-list step (the reference algorithm restated, OpenMP)
     on the same workload, from its state at t = t_mid (a step's cost grows as
@@ -188,12 +242,7 @@ def cpu_baseline(mid, t_mid, n, field, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     wall = time.perf_counter() - t0
-    model = ""
-    try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
-        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
-    except Exception:
-        pass
+    model = cpu_model()
     return {"value": n * steps / wall, "unit": "boid-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} full step(s) of the N={n} workload from its t={t_mid} state, oracle FP64 cell list, "
                       f"{threads} OpenMP threads, {wall:.1f} s wall; CPU: {model}"}
@@ -233,19 +282,23 @@ def valu_from_profiles(mode_name, n):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--workload", default="flock1m", choices=sorted(WORKLOADS),
+                    help="flock1m: the headline (N=2^20, F=4); flock16k: N=16384, F=1, FP64 plain vs the "
+                         "reference's own grid engine on the CPU; blobs16m: N=2^24, F=10, 64 clustered blobs")
+    ap.add_argument("--steps", type=int, default=None, help="timed updates (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--field", type=float, default=4.0)
-    ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--modes", default="grid_f64,grid_mixed")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--field", type=float, default=None)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--modes", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="neighbour-kernel workgroup size (0: automatic)")
     ap.add_argument("--tune", action="store_true",
                     help="pick block/tile per mode with the autotuner (default: engine defaults, which are the "
                          "autotuner's choice for this workload)")
-    ap.add_argument("--sweep-steps", type=int, default=100, help="batched sweep steps (0: skip the sweep)")
+    ap.add_argument("--sweep-steps", type=int, default=None,
+                    help="batched sweep steps (0: skip the sweep; default 100 with the headline workload)")
     ap.add_argument("--sweep-base-steps", type=int, default=10, help="steps of the per-flock baseline")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -254,12 +307,19 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    n, F = a.n, a.field
-    init = B.generate_init(E.INIT_ISOTROPIC, n, F, RULES["vmax"], RULES["vmin"], seed=a.seed)
+    wl = WORKLOADS[a.workload]
+    n = a.n if a.n is not None else wl["n"]
+    F = a.field if a.field is not None else wl["field"]
+    seed = a.seed if a.seed is not None else wl["seed"]
+    a.steps = a.steps if a.steps is not None else wl["steps"]
+    a.modes = a.modes or wl["modes"]
+    if a.sweep_steps is None:
+        a.sweep_steps = 100 if wl["sweep"] else 0
+    init = B.generate_init(wl["init"], n, F, RULES["vmax"], RULES["vmin"], seed=seed)
     by_name = {v: k for k, v in E.MODE_NAMES.items()}
-    t_mid = min(500, a.steps // 2)
+    t_mid = min(500, a.steps // 2) if wl["init"] == E.INIT_ISOTROPIC else 0
     mid = None
-    if a.tune or not a.no_cpu:
+    if a.tune or (not a.no_cpu and wl["cpu"] == "port"):
         log(f"[bench] mid-run state (t={t_mid}) for tuning and the CPU sample")
         mid = mid_run_state(init, n, F, t_mid)
     res = {}
@@ -270,7 +330,7 @@ def main():
         res[name]["tuning"] = tuning
         log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
 
-    head = res.get("grid_mixed") or next(iter(res.values()))
+    head = res.get(wl["head"]) or next(iter(res.values()))
     pair = head["kernels"].get("pair", {"us_avg": float("nan")})
     achieved = BYTES_PER_BOID_STEP * n / (pair["us_avg"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -282,11 +342,14 @@ def main():
         log(f"[bench] sweep: 512 flocks, {a.sweep_steps} batched steps, {a.sweep_base_steps} per-flock steps")
         sweep = run_sweep(a.sweep_steps, a.sweep_base_steps, 5, block=a.block)
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and wl["cpu"]:        # blobs16m: one oracle step alone exceeds the sample budget
         log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s)")
-        cpu = cpu_baseline(mid, t_mid, n, F, a.cpu_seconds)
+        cpu = (cpu_reference_grid(n, F, seed, a.cpu_seconds) if wl["cpu"] == "reference"
+               else cpu_baseline(mid, t_mid, n, F, a.cpu_seconds))
+    metric = ("boid-steps/sec at N=2^20: FP64 plain vs mixed precision; % HBM roofline" if a.workload == "flock1m"
+              else f"boid-steps/sec, {a.workload} (N={n}, {head['mode']})")
     out = {
-        "metric": "boid-steps/sec at N=2^20: FP64 plain vs mixed precision; % HBM roofline",
+        "metric": metric,
         "value": head["boid_steps_per_s"],
         "unit": "boid-steps/s",
         "n_gpus": 1,
@@ -297,11 +360,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if head["mode"] == "grid_mixed" else "f64",
-        "data": "synthetic (isotropic uniform init, seed %d)" % a.seed,
-        "config": {"workload": f"grid boid step, N={n}, F={F}, uniform positions, isotropic velocities "
-                               f"|v| in [0.001,0.005], radii 0.02/0.05/0.08, view 270 deg, weights "
-                               f"0.002/0.06/0.008, seed {a.seed}, timed updates t = 0..{a.steps - 1}",
-                   "n": n, "field": F, "timed_from_t": 0, "headline_mode": head["mode"], "parallelism": "single GPU"},
+        "data": f"synthetic ({wl['what']}, seed {seed})",
+        "config": {"workload": f"grid boid step, N={n}, F={F}, {wl['what']}, |v| in [0.001,0.005], radii "
+                               f"0.02/0.05/0.08, view 270 deg, weights 0.002/0.06/0.008, seed {seed}, timed updates "
+                               f"t = 0..{a.steps - 1}",
+                   "name": a.workload, "n": n, "field": F, "timed_from_t": 0, "headline_mode": head["mode"],
+                   "parallelism": "single GPU"},
         "fp64_plain": res.get("grid_f64"),
         "mixed": res.get("grid_mixed"),
         "mixed_over_fp64": (res["grid_mixed"]["boid_steps_per_s"] / res["grid_f64"]["boid_steps_per_s"]
