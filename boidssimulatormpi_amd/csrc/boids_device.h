@@ -191,12 +191,19 @@ __device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t cbase, 
 // ---------------------------------------------------------------------------
 // FP64 plain 27-cell kernel.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t grid)
+// Chunk of workgroup b, or nchunks (no work) for the grid's spare
+// workgroups.  Consecutive workgroups land on different XCDs (8 on
+// MI355X): each XCD gets a contiguous run of ceil(nchunks / 8) chunks, so
+// neighbouring rows share its L2 and every XCD gets an eighth of the work
+// (the grid is sized for the chunk table's capacity, which the actual
+// chunk count stays below).  Needs gridDim >= 8 * ceil(nchunks / 8).
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nchunks)
 {
-    // consecutive workgroups land on different XCDs (8 on MI355X); give
-    // each XCD a contiguous run of chunks so neighbouring rows share its L2.
-    const uint32_t per = grid >> 3;
-    return (b & 7u) * per + (b >> 3);
+    const uint32_t per = (nchunks + 7u) >> 3;
+    const uint32_t x = b & 7u, y = b >> 3;
+    if (y >= per) return nchunks;
+    const uint32_t c = x * per + y;
+    return c < nchunks ? c : nchunks;
 }
 
 // The oracle's FP64 decision for the pair (i, j): the minimum image is

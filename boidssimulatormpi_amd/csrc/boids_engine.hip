@@ -564,7 +564,9 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             // rows are cut into the fewest equal groups of <= kSpanMax cells
             const uint32_t ng = (uint32_t)((G + kSpanMax - 1) / kSpanMax);
             d.span = (uint32_t)((G + ng - 1) / ng);
-            e->groups += (uint64_t)G * G * ((G + d.span - 1) / d.span);
+            // chunk-table slack per span group: its last chunk, plus one chunk
+            // cut short per kChunkSpan cells (k_rows)
+            e->groups += (uint64_t)G * G * ((G + d.span - 1) / d.span) * (1 + (d.span - 1) / kChunkSpan);
             e->max_span = std::max(e->max_span, (int)d.span);
             cells += (uint64_t)G * G * G;
             rows += (uint64_t)G * G;

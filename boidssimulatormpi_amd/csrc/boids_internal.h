@@ -12,10 +12,15 @@ enum { SEP = 0, ALI = 1, COH = 2 };
 
 constexpr int kBlock = 256;        // threads per block of the streaming kernels
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
-// Neighbour kernels: a workgroup of NB threads (128 or 256) owns a chunk of
-// at most NB sorted boids inside one (y,z) row of cells, spanning at most
-// kSpanMax cells of that row.
+// Neighbour kernels: a workgroup of NB threads (64, 128 or 256) owns a
+// chunk of at most NB sorted boids inside one (y,z) row of cells; rows are
+// cut into span groups of at most kSpanMax cells, and a chunk covers at
+// most kChunkSpan cells.
 constexpr int kSpanMax = 62;
+#ifndef BS_CHUNK_SPAN
+#define BS_CHUNK_SPAN 16
+#endif
+constexpr int kChunkSpan = BS_CHUNK_SPAN;   // cells one neighbour-kernel chunk covers at most
 #ifndef BS_NBR_WAVES
 #define BS_NBR_WAVES 0             // >0: ask the compiler for at least this many waves per SIMD
 #endif

@@ -195,24 +195,28 @@ __global__ __launch_bounds__(kScanThreads) void k_rows(Flocks F, const uint32_t*
             span = fd.span;
             rl = r - fd.row_base;
             c0 = fd.cell_base + rl * G;
+        }
+        // Chunks of each span group: consecutive runs of `chunk` boids, cut
+        // short where a run would cover more than kChunkSpan cells (sparse
+        // stretches), so a chunk's tile tables stay small.
+        auto walk = [&](auto emit) {
             for (uint32_t x0 = 0; x0 < G; x0 += span) {
                 const uint32_t x1 = min(G, x0 + span);
-                const uint32_t cnt = start[c0 + x1] - start[c0 + x0];
-                kr += (cnt + chunk - 1) / chunk;
+                const uint32_t hi = start[c0 + x1];
+                uint32_t t0 = start[c0 + x0], xa = x0;
+                while (t0 < hi) {
+                    while (start[c0 + xa + 1] <= t0) ++xa;   // cell of boid t0
+                    uint32_t t1 = min(hi, t0 + chunk);
+                    if (xa + kChunkSpan < x1) t1 = min(t1, start[c0 + xa + kChunkSpan]);
+                    emit(t0, t1);
+                    t0 = t1;
+                }
             }
-        }
+        };
+        walk([&](uint32_t, uint32_t) { ++kr; });
         uint32_t total;
         uint32_t k = block_exclusive_scan(kr, wsum, &total) + carry;
-        if (kr) {
-            for (uint32_t x0 = 0; x0 < G; x0 += span) {
-                const uint32_t x1 = min(G, x0 + span);
-                const uint32_t lo = start[c0 + x0];
-                const uint32_t cnt = start[c0 + x1] - lo;
-                if (!cnt) continue;
-                for (uint32_t t0 = lo; t0 < lo + cnt; t0 += chunk)
-                    table[k++] = make_uint4(rl, t0, min(lo + cnt, t0 + chunk), f);
-            }
-        }
+        if (kr) walk([&](uint32_t t0, uint32_t t1) { table[k++] = make_uint4(rl, t0, t1, f); });
         carry += total;
     }
     if (threadIdx.x == 0) *nchunks = carry;

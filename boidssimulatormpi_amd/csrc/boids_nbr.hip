@@ -4,7 +4,7 @@
 //
 // A workgroup of NB threads owns a chunk [t0, t1) of sorted boids inside
 // one (y,z) row of cells, spanning cells cxa..cxb of that row (at most
-// kSpanMax).  Its tile is the 3 x 3 neighbour rows over the columns
+// kChunkSpan).  Its tile is the 3 x 3 neighbour rows over the columns
 // cxa-1 .. cxb+1, ordered column-major: entry e = 9 * column + rr, with
 // rr = 3 * (dz + 1) + (dy + 1), each entry the boids of one cell in sorted
 // order.  A boid in column L therefore finds its whole 27-cell window in the
@@ -186,8 +186,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
 
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= *nchunks) return;
+    const uint32_t nch = *nchunks;
+    const uint32_t b = xcd_chunk(blockIdx.x, nch);
+    if (b >= nch) return;
     const uint4 ch = table[b];
     const uint32_t row = __builtin_amdgcn_readfirstlane(ch.x);
     const uint32_t t0 = __builtin_amdgcn_readfirstlane(ch.y);
@@ -413,8 +414,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
 
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= *nchunks) return;
+    const uint32_t nch = *nchunks;
+    const uint32_t b = xcd_chunk(blockIdx.x, nch);
+    if (b >= nch) return;
     const uint4 ch = table[b];
     const uint32_t row = __builtin_amdgcn_readfirstlane(ch.x);
     const uint32_t t0 = __builtin_amdgcn_readfirstlane(ch.y);
@@ -712,7 +714,7 @@ hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in,
     if (F.n == 0) return hipSuccess;
     if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = tile_table_bytes(max_span);
+    const size_t dyn = tile_table_bytes(min(max_span, kChunkSpan));   // a chunk covers <= kChunkSpan cells
 #define BS_CALL(NB_, TD_) launch_f64_nb<NB_, TD_>(F, dyn, grid, in, rec, out, key_sorted, start, table, nchunks, \
                                                    key_out, rank, count, forces, digest, stream)
     BS_NBR_DISPATCH(nb, tile, BS_CALL);
@@ -729,7 +731,7 @@ hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_span, StateF64 i
     if (F.n == 0) return hipSuccess;
     if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = tile_table_bytes(max_span);
+    const size_t dyn = tile_table_bytes(min(max_span, kChunkSpan));   // a chunk covers <= kChunkSpan cells
 #define BS_CALL(NB_, TD_) launch_mixed_nb<NB_, TD_>(F, dyn, grid, in, out, lat, key_sorted, start, table, nchunks, \
                                                      key_out, rank, count, forces, digest, recheck, stream)
     BS_NBR_DISPATCH(nb, tile, BS_CALL);
