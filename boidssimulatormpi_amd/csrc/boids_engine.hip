@@ -40,6 +40,7 @@ struct bs_engine {
     uint32_t *key = nullptr, *key_sorted = nullptr, *count = nullptr, *start = nullptr;
     uint32_t *partials = nullptr, *nchunks = nullptr, *perm = nullptr, *permid = nullptr;
     uint32_t* rank = nullptr;          // arrival rank of each slot in its new cell
+    uint32_t *row_count = nullptr, *row_off = nullptr;   // chunks per (y,z) row, and their offsets
     uint4* table = nullptr;
     uint32_t max_chunks = 0, ncell1 = 0;
     // flock table (one entry for a single-flock engine)
@@ -626,6 +627,8 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->nchunks, 1)) ||
             (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) || (rc = dev_alloc(e, &e->rank, n)) ||
             (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)) ||
+            (rc = dev_alloc(e, &e->row_count, (size_t)e->rows + 1)) ||
+            (rc = dev_alloc(e, &e->row_off, (size_t)e->rows + 1)) ||
             (rc = dev_alloc(e, &e->flocks_dev, e->flocks.size())))
             return bail(rc);
         if (e->flocks.size() > 1) {
@@ -724,7 +727,8 @@ static int grid_sort_into_a(bs_engine* e)
 {
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
-    HIP_TRY(e, launch_rows(flock_view(e), e->start, e->table, e->nchunks, (int)e->chunk, e->stream));
+    HIP_TRY(e, launch_rows(flock_view(e), e->start, e->row_count, e->row_off, e->partials, e->table, e->nchunks,
+                           (int)e->chunk, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
     HIP_TRY(e, launch_scatter_ranked(e->prm.n, e->key, e->rank, e->start, e->db.id, e->perm, e->permid, e->stream));

@@ -175,51 +175,59 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
 // so that a chunk lies in one row, spans at most `span` cells and holds at
 // most `chunk` boids, and every wavefront but a group's last is full.
 // Entries are (local row, first slot, end slot, flock).  One workgroup.
-__global__ __launch_bounds__(kScanThreads) void k_rows(Flocks F, const uint32_t* start, uint4* table,
-                                                        uint32_t* nchunks, uint32_t chunk)
+// Chunk table, one wave per (y,z) row: the row's cell starts are read into
+// LDS (coalesced) and one lane walks them.  Each span group is cut into
+// consecutive runs of `chunk` boids, cut short where a run would cover more
+// than kChunkSpan cells (sparse stretches), so a chunk's tile tables stay
+// small.  Pass 1 counts the row's chunks; after a scan of the counts, pass 2
+// writes them at the row's offset (rows in order: neighbouring rows' chunks
+// stay adjacent in the table).
+constexpr int kRowWaves = 4;
+
+template <bool WRITE>
+__global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const uint32_t* __restrict__ start,
+                                                              uint32_t* row_count, const uint32_t* row_off,
+                                                              uint4* table, uint32_t* nchunks, uint32_t chunk)
 {
-    __shared__ uint32_t wsum[17];
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < F.rows; b += kScanThreads) {
-        const uint32_t r = b + threadIdx.x;
-        uint32_t f = 0, rl = 0, G = 0, span = 1, c0 = 0, kr = 0;
-        if (r < F.rows) {
-            uint32_t a = 0, z = F.count - 1;              // last flock with row_base <= r
-            while (a < z) {
-                const uint32_t m = (a + z + 1) >> 1;
-                if (F.fl[m].row_base <= r) a = m; else z = m - 1;
-            }
-            f = a;
-            const FlockDesc& fd = F.fl[f];
-            G = (uint32_t)fd.P.G;
-            span = fd.span;
-            rl = r - fd.row_base;
-            c0 = fd.cell_base + rl * G;
-        }
-        // Chunks of each span group: consecutive runs of `chunk` boids, cut
-        // short where a run would cover more than kChunkSpan cells (sparse
-        // stretches), so a chunk's tile tables stay small.
-        auto walk = [&](auto emit) {
-            for (uint32_t x0 = 0; x0 < G; x0 += span) {
-                const uint32_t x1 = min(G, x0 + span);
-                const uint32_t hi = start[c0 + x1];
-                uint32_t t0 = start[c0 + x0], xa = x0;
-                while (t0 < hi) {
-                    while (start[c0 + xa + 1] <= t0) ++xa;   // cell of boid t0
-                    uint32_t t1 = min(hi, t0 + chunk);
-                    if (xa + kChunkSpan < x1) t1 = min(t1, start[c0 + xa + kChunkSpan]);
-                    emit(t0, t1);
-                    t0 = t1;
-                }
-            }
-        };
-        walk([&](uint32_t, uint32_t) { ++kr; });
-        uint32_t total;
-        uint32_t k = block_exclusive_scan(kr, wsum, &total) + carry;
-        if (kr) walk([&](uint32_t t0, uint32_t t1) { table[k++] = make_uint4(rl, t0, t1, f); });
-        carry += total;
+    __shared__ uint32_t rs[kRowWaves][kMaxCells + 1];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t r = blockIdx.x * kRowWaves + w;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (WRITE) *nchunks = row_off[F.rows];
+        else row_count[F.rows] = 0u;            // the scan's total lands in row_off[rows]
     }
-    if (threadIdx.x == 0) *nchunks = carry;
+    if (r >= F.rows) return;                    // whole waves only: no barrier follows
+    uint32_t a = 0, z = F.count - 1;            // last flock with row_base <= r
+    while (a < z) {
+        const uint32_t m = (a + z + 1) >> 1;
+        if (F.fl[m].row_base <= r) a = m; else z = m - 1;
+    }
+    const uint32_t f = a;
+    const FlockDesc& fd = F.fl[f];
+    const uint32_t G = (uint32_t)fd.P.G, span = fd.span;
+    const uint32_t rl = r - fd.row_base;
+    const uint32_t c0 = fd.cell_base + rl * G;
+    uint32_t* cs = rs[w];
+    for (uint32_t x = lane; x <= G; x += 64) cs[x] = start[c0 + x];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+    uint32_t k = WRITE ? row_off[r] : 0u;
+    for (uint32_t x0 = 0; x0 < G; x0 += span) {
+        const uint32_t x1 = min(G, x0 + span);
+        const uint32_t hi = cs[x1];
+        uint32_t t0 = cs[x0], xa = x0;
+        while (t0 < hi) {
+            while (cs[xa + 1] <= t0) ++xa;          // cell of boid t0
+            uint32_t t1 = min(hi, t0 + chunk);
+            if (xa + kChunkSpan < x1) t1 = min(t1, cs[xa + kChunkSpan]);
+            if (WRITE) table[k] = make_uint4(rl, t0, t1, f);
+            ++k;
+            t0 = t1;
+        }
+    }
+    if (!WRITE) row_count[r] = k;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* key,
@@ -340,11 +348,17 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
     return hipGetLastError();
 }
 
-hipError_t launch_rows(Flocks F, const uint32_t* start, uint4* table, uint32_t* nchunks, int chunk,
-                       hipStream_t stream)
+hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, hipStream_t stream)
 {
-    if (F.count == 0 || chunk <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rows, dim3(1), dim3(kScanThreads), 0, stream, F, start, table, nchunks, (uint32_t)chunk);
+    if (F.count == 0 || chunk <= 0 || F.rows == 0) return hipErrorInvalidValue;
+    const dim3 grid((F.rows + kRowWaves - 1) / kRowWaves);
+    hipLaunchKernelGGL(k_rows_pass<false>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk);
+    hipError_t st = launch_scan(row_count, row_off, F.rows + 1, partials, stream);
+    if (st != hipSuccess) return st;
+    hipLaunchKernelGGL(k_rows_pass<true>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk);
     return hipGetLastError();
 }
 
