@@ -156,6 +156,8 @@ def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
     # run the flocks one after another, as the reference runs a sweep (one job
     # per parameter set, tools/gen_jobscript.py:92-111); per_flock_streams
     # interleaves all flocks' updates on their engines' own streams.
+    if base_steps <= 0:                 # batched only (no per-flock baselines)
+        return out
     for label, flags, interleave in (("per_flock", 0, False), ("per_flock_graph", E.FLAG_GRAPH, False),
                                      ("per_flock_streams", 0, True)):
         engines = []
@@ -299,7 +301,8 @@ def main():
                          "autotuner's choice for this workload)")
     ap.add_argument("--sweep-steps", type=int, default=None,
                     help="batched sweep steps (0: skip the sweep; default 100 with the headline workload)")
-    ap.add_argument("--sweep-base-steps", type=int, default=10, help="steps of the per-flock baseline")
+    ap.add_argument("--sweep-base-steps", type=int, default=10,
+                    help="steps of the per-flock baselines (0: batched sweep only)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the path's "
