@@ -540,7 +540,16 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
         e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? 9 : 6);
-        e->fill = p->fill ? p->fill : 100;
+        // Automatic fill: full chunks, unless that leaves fewer than ~2048
+        // of them (8 workgroups per CU) -- a small flock then runs in
+        // smaller chunks, more workgroups in flight (down to 25 %).
+        {
+            uint64_t nsum = 0;
+            for (uint32_t k = 0; k < count; ++k) nsum += fp[k].n;
+            const int nb_auto = p->block ? p->block : 64;
+            const uint64_t want = 100 * nsum / ((uint64_t)nb_auto * 2048);
+            e->fill = p->fill ? p->fill : (int)std::max<uint64_t>(25, std::min<uint64_t>(100, want));
+        }
         uint64_t cells = 0, rows = 0;
         bool coarse = false;
         for (uint32_t k = 0; k < count; ++k) {

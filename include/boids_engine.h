@@ -89,7 +89,8 @@ typedef struct {
     int32_t tile;                /* grid modes: LDS tile depth, candidates staged per thread and pass:
                                   * 6, 9 or 12; 0 = automatic (6 FP64, 9 mixed) */
     int32_t fill;                /* grid modes: most boids per chunk, percent of the workgroup size
-                                  * (25..100); 0 = automatic (100) */
+                                  * (25..100); 0 = automatic (100, less for small flocks: about 2048
+                                  * chunks at least) */
 } bs_params;
 
 /* Read-only facts about a created engine. */
