@@ -100,9 +100,14 @@ def tuned_params(mode, mid, n, field, tune, block=0):
 def run_mode(mode, init, n, field, steps, warmup, params):
     e = B.Engine(params)
     info = e.info()
-    e.set_state(xyzv6=init)
+    def load():
+        if isinstance(init, tuple):
+            e.set_state(pos=init[0], vel=init[1])
+        else:
+            e.set_state(xyzv6=init)
+    load()
     e.step(warmup)
-    e.set_state(xyzv6=init)          # the timed steps are the run from t = 0
+    load()                           # the timed steps are the run from the start state
     e.sync()
     e.reset_timing()
     t0 = time.perf_counter()
@@ -272,13 +277,17 @@ def valu_from_profiles(mode_name, n):
     got = {}
     with open(path) as f:
         for line in f:
-            for key in ("valu_issue", "active_lanes"):
+            for key in ("valu_issue", "active_lanes", "valu_issue_t500", "active_lanes_t500"):
                 if line.startswith(f"{key}_{mode_name}:"):
                     got[key] = float(line.split(":", 1)[1])
-    if len(got) != 2:
+    if "valu_issue" not in got or "active_lanes" not in got:
         return None
-    return {"issue_frac": got["valu_issue"], "active_lanes": got["active_lanes"],
-            "source": os.path.relpath(path, ROOT), "launches": "first 22 updates (uniform start)"}
+    out = {"issue_frac": got["valu_issue"], "active_lanes": got["active_lanes"],
+           "source": os.path.relpath(path, ROOT), "launches": "updates t = 0..21 (uniform start)"}
+    if "valu_issue_t500" in got and "active_lanes_t500" in got:
+        out["clustered"] = {"issue_frac": got["valu_issue_t500"], "active_lanes": got["active_lanes_t500"],
+                            "launches": "updates t = 500..521"}
+    return out
 
 
 def main():
@@ -289,6 +298,9 @@ def main():
                          "reference's own grid engine on the CPU; blobs16m: N=2^24, F=10, 64 clustered blobs")
     ap.add_argument("--steps", type=int, default=None, help="timed updates (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--from-t", type=int, default=0,
+                    help="start the timed updates from the workload's state after this many updates (FP64 "
+                         "engine, untimed; default 0: the run from its initial state)")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--field", type=float, default=None)
     ap.add_argument("--seed", type=int, default=None)
@@ -326,10 +338,14 @@ def main():
         log(f"[bench] mid-run state (t={t_mid}) for tuning and the CPU sample")
         mid = mid_run_state(init, n, F, t_mid)
     res = {}
+    start = init
+    if a.from_t > 0:
+        log(f"[bench] start state: t={a.from_t}")
+        start = mid_run_state(init, n, F, a.from_t)
     for name in a.modes.split(","):
         log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} steps={a.steps}")
         params, tuning = tuned_params(by_name[name], mid, n, F, a.tune, a.block)
-        res[name] = run_mode(by_name[name], init, n, F, a.steps, a.warmup, params)
+        res[name] = run_mode(by_name[name], start, n, F, a.steps, a.warmup, params)
         res[name]["tuning"] = tuning
         log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
 
@@ -366,8 +382,8 @@ def main():
         "data": f"synthetic ({wl['what']}, seed {seed})",
         "config": {"workload": f"grid boid step, N={n}, F={F}, {wl['what']}, |v| in [0.001,0.005], radii "
                                f"0.02/0.05/0.08, view 270 deg, weights 0.002/0.06/0.008, seed {seed}, timed updates "
-                               f"t = 0..{a.steps - 1}",
-                   "name": a.workload, "n": n, "field": F, "timed_from_t": 0, "headline_mode": head["mode"],
+                               f"t = {a.from_t}..{a.from_t + a.steps - 1}",
+                   "name": a.workload, "n": n, "field": F, "timed_from_t": a.from_t, "headline_mode": head["mode"],
                    "parallelism": "single GPU"},
         "fp64_plain": res.get("grid_f64"),
         "mixed": res.get("grid_mixed"),
