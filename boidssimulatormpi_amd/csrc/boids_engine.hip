@@ -575,7 +575,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             const uint32_t ng = (uint32_t)((G + kSpanMax - 1) / kSpanMax);
             d.span = (uint32_t)((G + ng - 1) / ng);
             // chunk-table slack per span group: its last chunk, plus one chunk
-            // cut short per kChunkSpan cells (k_rows)
+            // cut short per kChunkSpan cells (k_rows_pass)
             e->groups += (uint64_t)G * G * ((G + d.span - 1) / d.span) * (1 + (d.span - 1) / kChunkSpan);
             e->max_span = std::max(e->max_span, (int)d.span);
             cells += (uint64_t)G * G * G;
@@ -627,7 +627,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     if ((rc = dev_alloc(e, &e->tmp_pos, 3 * (size_t)n)) || (rc = dev_alloc(e, &e->tmp_vel, 3 * (size_t)n))) return bail(rc);
     if (p->flags & BS_FLAG_DIGEST) { if ((rc = dev_alloc(e, &e->digest, 4 * (size_t)n))) return bail(rc); }
     if (is_grid(e->mode)) {
-        // sum over groups of ceil(n_group / chunk) <= n / chunk + groups
+        // full chunks <= n / chunk; groups counts each (row, span group)'s last chunk and short cuts (k_rows_pass)
         if ((uint64_t)(n + e->chunk - 1) / e->chunk + e->groups + 1 > 0x7fffffffull) return bail(fail(e, BS_E_ARG, "chunk table too large"));
         e->max_chunks = (uint32_t)((n + e->chunk - 1) / e->chunk + e->groups + 1);
         const uint32_t ntiles = (e->ncell1 + 4095) / 4096;

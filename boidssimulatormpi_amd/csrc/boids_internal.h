@@ -120,7 +120,7 @@ hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t
 hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,
                          const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
                          uint32_t* key_sorted, Lattice* lat, double2* rec, hipStream_t stream);
-// Neighbour kernels (boids_nbr.hip).  nb = workgroup size (128 or 256),
+// Neighbour kernels (boids_nbr.hip).  nb = workgroup size (64, 128 or 256),
 // tile = LDS candidates per thread and pass (6, 9 or 12), max_span =
 // largest FlockDesc::span, max_chunks = capacity of `table`.
 hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in, const double2* rec,

@@ -169,19 +169,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
     }
 }
 
-// Block table of the neighbour kernels.  Every (y,z) row of cells of every
-// flock is cut into groups of `span` consecutive cells, and every group
-// into chunks of `chunk` consecutive sorted boids (the last one shorter),
-// so that a chunk lies in one row, spans at most `span` cells and holds at
-// most `chunk` boids, and every wavefront but a group's last is full.
-// Entries are (local row, first slot, end slot, flock).  One workgroup.
-// Chunk table, one wave per (y,z) row: the row's cell starts are read into
-// LDS (coalesced) and one lane walks them.  Each span group is cut into
-// consecutive runs of `chunk` boids, cut short where a run would cover more
-// than kChunkSpan cells (sparse stretches), so a chunk's tile tables stay
-// small.  Pass 1 counts the row's chunks; after a scan of the counts, pass 2
-// writes them at the row's offset (rows in order: neighbouring rows' chunks
-// stay adjacent in the table).
+// Chunk table of the neighbour kernels, entries (local row, first slot, end
+// slot, flock).  Every (y,z) row of cells of every flock is cut into groups
+// of `span` consecutive cells, and every group into runs of `chunk`
+// consecutive sorted boids, cut short where a run would cover more than
+// kChunkSpan cells (sparse stretches), so a chunk lies in one row, holds at
+// most `chunk` boids and its tile tables stay small.  A short cut moves the
+// run's first cell on by at least kChunkSpan, so a group has at most
+// (span - 1) / kChunkSpan of them: the table holds at most n / chunk +
+// groups * (1 + (span - 1) / kChunkSpan) entries.  One wave per row: the
+// row's cell starts are read into LDS (coalesced) and one lane walks them.
+// Pass 1 counts the row's chunks; after a scan of the counts, pass 2 writes
+// them at the row's offset (rows in order: neighbouring rows' chunks stay
+// adjacent in the table).
 constexpr int kRowWaves = 4;
 
 template <bool WRITE>
