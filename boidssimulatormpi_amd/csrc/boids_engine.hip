@@ -4,6 +4,15 @@
 #include "boids_engine.h"
 #include "boids_internal.h"
 
+// Automatic LDS tile depth of the neighbour kernels (candidates per thread
+// and pass): 6, 9 or 12.
+#ifndef BS_MIXED_TILE
+#define BS_MIXED_TILE 6
+#endif
+#ifndef BS_F64_TILE
+#define BS_F64_TILE 6
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -539,7 +548,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         }
         if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
-        e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? 9 : 6);
+        e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_TILE : BS_F64_TILE);
         // Automatic fill: full chunks, unless that leaves fewer than ~2048
         // of them (8 workgroups per CU) -- a small flock then runs in
         // smaller chunks, more workgroups in flight (down to 25 %).

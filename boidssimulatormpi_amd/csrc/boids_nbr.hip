@@ -123,17 +123,26 @@ __device__ __forceinline__ float lane_read(float v, uint32_t src)
     return __builtin_bit_cast(float, lane_read(__builtin_bit_cast(uint32_t, v), src));
 }
 
-// Lanes of this wave whose bit is set in m below the calling lane.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+// Mask of the wave's lanes where c holds, taken from the compare's own lane
+// mask (__ballot and __any take an int, which costs a select and a second
+// compare per call).
+__device__ __forceinline__ uint64_t wave_mask(bool c)
 {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    return __builtin_amdgcn_ballot_w64(c);
+}
+
+// base + the number of lanes of this wave below the calling lane whose bit
+// is set in m (base, wave-uniform, is mbcnt's accumulator).
+__device__ __forceinline__ uint32_t lanes_below_plus(uint64_t m, uint32_t base)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
 }
 
 // Per-wave hit list of the neighbour kernels, entries (tile entry << 22) |
-// (tile index << 6) | owner lane, appended by ballot and drained in full
-// rounds of 64 (kRounds rounds per step) once it holds kDrain entries.  The
-// list has 64 more slots where lanes without a hit write (one per lane), so
-// appends need no branches.
+// (tile index << 6) | owner lane, appended by ballot (each hit stored under
+// its own lane mask) and drained in full rounds of 64 (kRounds rounds per
+// step) once it holds kDrain entries.  A filter step appends at most 4 x 64
+// entries to a list holding fewer than kDrain, so kCap bounds every index.
 #ifndef BS_LIST_DRAIN
 #define BS_LIST_DRAIN 128
 #endif
@@ -177,7 +186,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     // coarse positions, tile entry in bits 22-31 of .y; + 4: a finished
     // lane's look-ahead reads (positions stay <= base + here)
     __shared__ uint2 tco[Cfg::kCap + 4];
-    __shared__ uint32_t wl[NW][kListCap + 64];
+    __shared__ uint32_t wl[NW][kListCap];
     __shared__ double2 own_a[NB], own_b[NB], own_c[NB];   // owner (x, y), (z, vx), (vy, vz)
     __shared__ double own_n[NB];                // owner |v|
     __shared__ double acc[9][NB];               // sums of q (cohesion), e/|e| (separation), v (alignment)
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         uint32_t tail = 0;
         uint32_t k = kb;
         uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
-        while (__any(k < ke)) {
+        while (wave_mask(k < ke) != 0) {
             const uint32_t kn = min(k + 4, ke);
             const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
             const uint32_t left = ke - k;
@@ -323,21 +332,23 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
             };
             const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
             const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
-            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
-            const uint32_t n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popcll(m3);
+            const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
+            const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
+            const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
+            const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
+            const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
             const uint32_t e = ((k - base) << 6) | lane;
-            const uint32_t trash = kListCap + lane;
-            list[h0 ? tail + lanes_below(m0) : trash] = (a0.y & 0xFFC00000u) | e;
-            list[h1 ? tail + n0 + lanes_below(m1) : trash] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
-            list[h2 ? tail + n0 + n1 + lanes_below(m2) : trash] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
-            list[h3 ? tail + n0 + n1 + n2 + lanes_below(m3) : trash] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
-            tail += n0 + n1 + n2 + n3;
+            if (h0) list[lanes_below_plus(m0, tail)] = (a0.y & 0xFFC00000u) | e;
+            if (h1) list[lanes_below_plus(m1, t1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
+            if (h2) list[lanes_below_plus(m2, t2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
+            if (h3) list[lanes_below_plus(m3, t3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+            tail = t3 + __popcll(m3);
             if (tail >= kListDrain) {
                 const uint32_t lim = tail - tail % kStep, rem = tail - lim;
                 drain(lim, base);
                 uint32_t mv[kRounds];
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + 64 * i + lane];
+                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= tail < kListCap
 #pragma unroll
                 for (uint32_t i = 0; i < kRounds; ++i)
                     if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
@@ -407,7 +418,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
     // + 4: a finished lane's look-ahead reads
     __shared__ uint2 tco[Cfg::kCap + 4];
-    __shared__ uint32_t wl[NW][kListCap + 64];
+    __shared__ uint32_t wl[NW][kListCap];
     __shared__ double acc[9][NB];               // integer-valued sums: cohesion, separation, alignment
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
@@ -598,31 +609,35 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         uint32_t tail = 0;
         uint32_t k = kb;
         uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
-        while (__any(k < ke)) {
+        while (wave_mask(k < ke) != 0) {
             const uint32_t kn = min(k + 4, ke);
             const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
             const uint32_t left = ke - k;
             auto near = [&](uint2 q) {
                 return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= P.coarse_thr;
             };
+            // masks straight from the compares (scalar ANDs); each hit is
+            // stored under its own lane mask
             const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
             const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
-            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
-            const uint32_t n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popcll(m3);
+            const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
+            const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
+            const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
+            const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
+            const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
             const uint32_t e = ((k - base) << 6) | lane;
-            const uint32_t trash = kListCap + lane;
-            list[h0 ? tail + lanes_below(m0) : trash] = (a0.y & 0xFFC00000u) | e;
-            list[h1 ? tail + n0 + lanes_below(m1) : trash] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
-            list[h2 ? tail + n0 + n1 + lanes_below(m2) : trash] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
-            list[h3 ? tail + n0 + n1 + n2 + lanes_below(m3) : trash] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
-            tail += n0 + n1 + n2 + n3;
+            if (h0) list[lanes_below_plus(m0, tail)] = (a0.y & 0xFFC00000u) | e;
+            if (h1) list[lanes_below_plus(m1, t1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
+            if (h2) list[lanes_below_plus(m2, t2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
+            if (h3) list[lanes_below_plus(m3, t3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+            tail = t3 + __popcll(m3);
             if (tail >= kListDrain) {
                 // Phase 2 on every full step of rounds; the rest moves to the front.
                 const uint32_t lim = tail - tail % kStep, rem = tail - lim;
                 drain(lim, base);
                 uint32_t mv[kRounds];
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + 64 * i + lane];
+                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= tail < kListCap
 #pragma unroll
                 for (uint32_t i = 0; i < kRounds; ++i)
                     if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
