@@ -131,11 +131,10 @@ __device__ __forceinline__ uint64_t wave_mask(bool c)
     return __builtin_amdgcn_ballot_w64(c);
 }
 
-// base + the number of lanes of this wave below the calling lane whose bit
-// is set in m (base, wave-uniform, is mbcnt's accumulator).
-__device__ __forceinline__ uint32_t lanes_below_plus(uint64_t m, uint32_t base)
+// Lanes of this wave below the calling lane whose bit is set in m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Per-wave hit list of the neighbour kernels, entries (tile entry << 22) |
@@ -150,7 +149,7 @@ __device__ __forceinline__ uint32_t lanes_below_plus(uint64_t m, uint32_t base)
 #define BS_MIXED_ROUNDS 4
 #endif
 #ifndef BS_F64_ROUNDS
-#define BS_F64_ROUNDS 2
+#define BS_F64_ROUNDS 4
 #endif
 template <uint32_t R> struct ListCfg {
     static constexpr uint32_t kRounds = R;                // rounds of 64 pairs evaluated per drain step
@@ -159,6 +158,51 @@ template <uint32_t R> struct ListCfg {
     static constexpr uint32_t kCap = kDrain + 4 * 64;     // + one filter step (4 candidates per lane)
     static_assert(kDrain % kStep == 0, "list drain threshold");
 };
+
+// Phase 1 of the neighbour kernels over one lane's window [k, k1) of the
+// staged tile (positions relative to the pass's first): four candidates per
+// lane and step, the next four read ahead (positions stay <= k1 + 3, inside
+// the tile's 4 entries of padding); the list is flushed whenever it holds
+// kDrain entries, so a step always starts below kDrain and appends at most
+// 4 x 64 more.  A candidate within the coarse bound is appended as
+// (tile entry << 22) | (tile position << 6) | lane, in (step, candidate,
+// lane) order; the owner itself is a hit (coarse distance 0) and is skipped
+// by the evaluation.  flush(tail) evaluates the list's full rounds and
+// returns the number of entries it leaves at the front.  Returns the final
+// tail.  Every lane of the wave takes part.
+template <uint32_t kDrain, typename FLUSH>
+__device__ __forceinline__ uint32_t filter_window(const uint2* tco, uint32_t* list, uint32_t k, uint32_t k1,
+                                                  uint32_t lane, v2s my_xy, v2s my_z, int32_t thr, FLUSH&& flush)
+{
+    auto near = [&](uint2 q) { return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= thr; };
+    uint32_t tail = 0;
+    uint2 a0 = tco[k], a1 = tco[k + 1], a2 = tco[k + 2], a3 = tco[k + 3];
+    while (wave_mask(k < k1) != 0) {
+        const uint32_t kn = min(k + 4, k1);
+        const uint2 b0 = tco[kn], b1 = tco[kn + 1], b2 = tco[kn + 2], b3 = tco[kn + 3];
+        // masks straight from the compares (scalar ANDs); each hit is
+        // stored under its own lane mask
+        const uint32_t left = k1 - k;
+        const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
+        const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
+        const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
+        const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
+        const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
+        const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
+        const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
+        const uint32_t e = (k << 6) | lane;
+        // the wave-uniform list offsets go into the (scalar) base address
+        if (h0) (list + tail)[lanes_below(m0)] = (a0.y & 0xFFC00000u) | e;
+        if (h1) (list + t1)[lanes_below(m1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
+        if (h2) (list + t2)[lanes_below(m2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
+        if (h3) (list + t3)[lanes_below(m3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+        tail = t3 + __popcll(m3);
+        if (tail >= kDrain) tail = flush(tail);
+        k = kn;
+        a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+    }
+    return tail;
+}
 
 // ---------------------------------------------------------------------------
 // FP64 plain.
@@ -185,11 +229,17 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     constexpr int NW = NB / 64;
     // coarse positions, tile entry in bits 22-31 of .y; + 4: a finished
     // lane's look-ahead reads (positions stay <= base + here)
-    __shared__ uint2 tco[Cfg::kCap + 4];
+    // the tile first, at LDS address 0 (the largest variable is placed
+    // first), so phase 1's reads need no base-address additions
+    __shared__ struct {
+        uint2 tco[Cfg::kCap + 4];
+        double acc[9][NB];                      // sums of q (cohesion), e/|e| (separation), v (alignment)
+    } sh;
+    auto& tco = sh.tco;
+    auto& acc = sh.acc;
     __shared__ uint32_t wl[NW][kListCap];
     __shared__ double2 own_a[NB], own_b[NB], own_c[NB];   // owner (x, y), (z, vx), (vy, vz)
     __shared__ double own_n[NB];                // owner |v|
-    __shared__ double acc[9][NB];               // sums of q (cohesion), e/|e| (separation), v (alignment)
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
@@ -245,7 +295,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
-    uint32_t* list = wl[tid >> 6];
+    uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
 
     // One (owner, candidate) pair per lane: the candidate's record from L2
     // by slot, read ahead of the evaluation.
@@ -295,7 +345,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
             if (in_a) atomicAdd(&acc_h[2 * NB + o], h);
         }
     };
-    auto drain = [&](uint32_t lim, uint32_t base) {
+    auto drain = [&](uint32_t lim, uint32_t base) __attribute__((always_inline)) {
         Rec c[kRounds];
 #pragma unroll
         for (uint32_t i = 0; i < kRounds; ++i) c[i] = load(64 * i + lane, lim, base);
@@ -320,43 +370,19 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         __syncthreads();
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
-        uint32_t tail = 0;
-        uint32_t k = kb;
-        uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
-        while (wave_mask(k < ke) != 0) {
-            const uint32_t kn = min(k + 4, ke);
-            const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
-            const uint32_t left = ke - k;
-            auto near = [&](uint2 q) {
-                return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= P.coarse_thr;
-            };
-            const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
-            const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
-            const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
-            const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
-            const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
-            const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
-            const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
-            const uint32_t e = ((k - base) << 6) | lane;
-            if (h0) list[lanes_below_plus(m0, tail)] = (a0.y & 0xFFC00000u) | e;
-            if (h1) list[lanes_below_plus(m1, t1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
-            if (h2) list[lanes_below_plus(m2, t2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
-            if (h3) list[lanes_below_plus(m3, t3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
-            tail = t3 + __popcll(m3);
-            if (tail >= kListDrain) {
-                const uint32_t lim = tail - tail % kStep, rem = tail - lim;
-                drain(lim, base);
-                uint32_t mv[kRounds];
+        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+                                                        [&](uint32_t t) __attribute__((always_inline)) {
+            // phase 2 on every full step of rounds; the rest moves to the front
+            const uint32_t lim = t - t % kStep, rem = t - lim;
+            drain(lim, base);
+            uint32_t mv[kRounds];
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= tail < kListCap
+            for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= t < kListCap
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i)
-                    if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
-                tail = rem;
-            }
-            k = kn;
-            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-        }
+            for (uint32_t i = 0; i < kRounds; ++i)
+                if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
+            return rem;
+        });
         if (tail) drain(tail, base);
     }
     if (!active) return;
@@ -417,9 +443,15 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     constexpr int NW = NB / 64;
     // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
     // + 4: a finished lane's look-ahead reads
-    __shared__ uint2 tco[Cfg::kCap + 4];
+    // the tile first, at LDS address 0 (the largest variable is placed
+    // first), so phase 1's reads need no base-address additions
+    __shared__ struct {
+        uint2 tco[Cfg::kCap + 4];
+        double acc[9][NB];                      // integer-valued sums: cohesion, separation, alignment
+    } sh;
+    auto& tco = sh.tco;
+    auto& acc = sh.acc;
     __shared__ uint32_t wl[NW][kListCap];
-    __shared__ double acc[9][NB];               // integer-valued sums: cohesion, separation, alignment
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
@@ -481,7 +513,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
-    uint32_t* list = wl[tid >> 6];
+    uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
     constexpr double kSub = 1.0 / (double)(1 << kSubBits);
 
     // Adds the pair (owner o at lattice m, candidate record q, w, slot j)
@@ -573,7 +605,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     };
     // Evaluates list[0, lim) in rounds of 64 lanes, kRounds rounds per
     // step, the next kRounds fetched while these are evaluated.
-    auto drain = [&](uint32_t lim, uint32_t base) {
+    auto drain = [&](uint32_t lim, uint32_t base) __attribute__((always_inline)) {
         Rec c[kRounds];
 #pragma unroll
         for (uint32_t i = 0; i < kRounds; ++i) c[i] = load(list, 64 * i + lane, lim, base);
@@ -602,50 +634,20 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         // reads stay inside tco
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
-        // Phase 1: four candidates per lane and step, the next four read
-        // ahead; the wave's hits are appended to its list in (step,
-        // candidate, lane) order.  The owner itself is a hit (coarse
-        // distance 0) and is skipped by the evaluation.
-        uint32_t tail = 0;
-        uint32_t k = kb;
-        uint2 a0 = tco[k - base], a1 = tco[k - base + 1], a2 = tco[k - base + 2], a3 = tco[k - base + 3];
-        while (wave_mask(k < ke) != 0) {
-            const uint32_t kn = min(k + 4, ke);
-            const uint2 b0 = tco[kn - base], b1 = tco[kn - base + 1], b2 = tco[kn - base + 2], b3 = tco[kn - base + 3];
-            const uint32_t left = ke - k;
-            auto near = [&](uint2 q) {
-                return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= P.coarse_thr;
-            };
-            // masks straight from the compares (scalar ANDs); each hit is
-            // stored under its own lane mask
-            const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
-            const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
-            const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
-            const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
-            const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
-            const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
-            const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
-            const uint32_t e = ((k - base) << 6) | lane;
-            if (h0) list[lanes_below_plus(m0, tail)] = (a0.y & 0xFFC00000u) | e;
-            if (h1) list[lanes_below_plus(m1, t1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
-            if (h2) list[lanes_below_plus(m2, t2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
-            if (h3) list[lanes_below_plus(m3, t3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
-            tail = t3 + __popcll(m3);
-            if (tail >= kListDrain) {
-                // Phase 2 on every full step of rounds; the rest moves to the front.
-                const uint32_t lim = tail - tail % kStep, rem = tail - lim;
-                drain(lim, base);
-                uint32_t mv[kRounds];
+        // phase 1 (filter_window), phase 2 whenever the list is full
+        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+                                                        [&](uint32_t t) __attribute__((always_inline)) {
+            // phase 2 on every full step of rounds; the rest moves to the front
+            const uint32_t lim = t - t % kStep, rem = t - lim;
+            drain(lim, base);
+            uint32_t mv[kRounds];
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= tail < kListCap
+            for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= t < kListCap
 #pragma unroll
-                for (uint32_t i = 0; i < kRounds; ++i)
-                    if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
-                tail = rem;
-            }
-            k = kn;
-            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-        }
+            for (uint32_t i = 0; i < kRounds; ++i)
+                if (64 * i + lane < rem) list[64 * i + lane] = mv[i];
+            return rem;
+        });
         if (tail) drain(tail, base);
     }
     if (nre) atomicAdd(recheck, nre);
