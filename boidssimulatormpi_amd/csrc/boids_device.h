@@ -88,10 +88,13 @@ __device__ __forceinline__ uint32_t axis_cell(double x, double inv_edge, int G)
     return (uint32_t)c;
 }
 
+// Cell key: x column (sx per cell edge; sx = 1 or 2, so sx * inv_edge is
+// exactly sx G / F) + sx G * (y cell + G * z cell).
 __device__ __forceinline__ uint32_t cell_of(const DevParams& P, double x, double y, double z)
 {
-    uint32_t G = (uint32_t)P.G;
-    return axis_cell(x, P.inv_edge, P.G) + G * (axis_cell(y, P.inv_edge, P.G) + G * axis_cell(z, P.inv_edge, P.G));
+    const uint32_t G = (uint32_t)P.G, Gx = (uint32_t)P.sx * G;
+    return axis_cell(x, (double)P.sx * P.inv_edge, (int)Gx) +
+           Gx * (axis_cell(y, P.inv_edge, P.G) + G * axis_cell(z, P.inv_edge, P.G));
 }
 
 // Exclusive scan of one value per thread across the block (blockDim a

@@ -12,6 +12,18 @@
 #ifndef BS_F64_TILE
 #define BS_F64_TILE 6
 #endif
+// x columns per cell edge of the grid modes (DevParams::sx): the mixed path
+// bins x at half the cell edge (a 2.5-edge window along x instead of 3);
+// the FP64 path keeps the reference's cells, where the shorter chunks that
+// finer columns bring cost more than the smaller windows save.
+#ifndef BS_MIXED_SUBX
+#define BS_MIXED_SUBX 2
+#endif
+#ifndef BS_F64_SUBX
+#define BS_F64_SUBX 1
+#endif
+static_assert(BS_MIXED_SUBX >= 1 && BS_MIXED_SUBX <= boids::kSubXMax && BS_F64_SUBX >= 1 &&
+                  BS_F64_SUBX <= boids::kSubXMax, "x columns per cell edge");
 
 #include <float.h>
 #include <math.h>
@@ -60,6 +72,7 @@ struct bs_engine {
     uint32_t rows = 0;
     uint64_t groups = 0;               // (row, span group) pairs over all flocks
     int max_g = 0, max_span = 0;
+    int sx = 1;                        // x columns per cell edge of every flock (grid modes)
     int nb = 64;                       // neighbour-kernel workgroup size
     int tile = 12;                     // LDS tile depth (candidates per thread and pass)
     int fill = 100;                    // chunk size target, percent of nb
@@ -576,18 +589,21 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
                 break;
             }
             d.P.G = G;
+            d.P.sx = e->sx = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_SUBX : BS_F64_SUBX;
             d.P.inv_edge = G / q.field;
             d.cell_base = (uint32_t)cells;
             d.row_base = (uint32_t)rows;
             d.nrows = (uint32_t)G * (uint32_t)G;
-            // rows are cut into the fewest equal groups of <= kSpanMax cells
-            const uint32_t ng = (uint32_t)((G + kSpanMax - 1) / kSpanMax);
-            d.span = (uint32_t)((G + ng - 1) / ng);
+            // rows (sx G x columns) are cut into the fewest equal groups of
+            // <= kSpanMax columns
+            const uint32_t Gx = (uint32_t)d.P.sx * (uint32_t)G;
+            const uint32_t ng = (Gx + kSpanMax - 1) / kSpanMax;
+            d.span = (Gx + ng - 1) / ng;
             // chunk-table slack per span group: its last chunk, plus one chunk
-            // cut short per kChunkSpan cells (k_rows_pass)
-            e->groups += (uint64_t)G * G * ((G + d.span - 1) / d.span) * (1 + (d.span - 1) / kChunkSpan);
+            // cut short per kChunkSpan columns (k_rows_pass)
+            e->groups += (uint64_t)G * G * ((Gx + d.span - 1) / d.span) * (1 + (d.span - 1) / kChunkSpan);
             e->max_span = std::max(e->max_span, (int)d.span);
-            cells += (uint64_t)G * G * G;
+            cells += (uint64_t)Gx * G * G;
             rows += (uint64_t)G * G;
             if (!(d.P.unit * 8.0 < 1e-6)) coarse = true;   // lattice too coarse for the mixed guard bands
             e->flocks.push_back(d);
@@ -609,6 +625,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             e->G = e->flocks[0].P.G;
             e->edge = fp[0].field / e->G;
             e->P.G = e->G;
+            e->P.sx = e->sx;
             e->P.inv_edge = e->flocks[0].P.inv_edge;
             if (e->mode == BS_MODE_GRID_MIXED && coarse) e->mode = BS_MODE_GRID_F64;
         } else {
@@ -929,10 +946,10 @@ static int grid_step(bs_engine* e)
     Timed t(e, BS_K_PAIR);
     const Flocks F = flock_view(e);
     if (e->mode == BS_MODE_GRID_F64)
-        HIP_TRY(e, launch_nbr_f64(F, e->nb, e->tile, e->max_span, e->da, e->rec, e->db, e->key_sorted, e->start, e->table, e->nchunks,
+        HIP_TRY(e, launch_nbr_f64(F, e->nb, e->tile, std::min(e->max_span, kChunkSpan) + 2 * e->sx, e->da, e->rec, e->db, e->key_sorted, e->start, e->table, e->nchunks,
                                   e->max_chunks, e->key, e->rank, e->count, (double*)e->forces, e->digest, e->stream));
     else
-        HIP_TRY(e, launch_nbr_mixed(F, e->nb, e->tile, e->max_span, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
+        HIP_TRY(e, launch_nbr_mixed(F, e->nb, e->tile, std::min(e->max_span, kChunkSpan) + 2 * e->sx, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
                                     e->nchunks, e->max_chunks, e->key, e->rank, e->count, (double*)e->forces,
                                     e->digest, e->recheck, e->stream));
     t.stop();

@@ -12,15 +12,21 @@ enum { SEP = 0, ALI = 1, COH = 2 };
 
 constexpr int kBlock = 256;        // threads per block of the streaming kernels
 constexpr int kMaxCells = 1024;    // cells per axis cap of the grid modes
+// Grid modes bin x into DevParams::sx columns per cell edge (1 or 2;
+// columns of edge/sx, y and z keep the cell edge >= the largest radius): a
+// boid's neighbours lie within sx columns on either side, so its window is
+// (2 sx + 1) columns x 3 x 3 rows -- with sx = 2, 2.5 cell edges along x
+// instead of 3.  kSubXMax bounds sx.
+constexpr int kSubXMax = 2;
 // Neighbour kernels: a workgroup of NB threads (64, 128 or 256) owns a
 // chunk of at most NB sorted boids inside one (y,z) row of cells; rows are
-// cut into span groups of at most kSpanMax cells, and a chunk covers at
-// most kChunkSpan cells.
+// cut into span groups of at most kSpanMax x columns, and a chunk covers at
+// most kChunkSpan columns.
 constexpr int kSpanMax = 62;
 #ifndef BS_CHUNK_SPAN
 #define BS_CHUNK_SPAN 16
 #endif
-constexpr int kChunkSpan = BS_CHUNK_SPAN;   // cells one neighbour-kernel chunk covers at most
+constexpr int kChunkSpan = BS_CHUNK_SPAN;   // x columns one neighbour-kernel chunk covers at most
 #ifndef BS_NBR_WAVES
 #define BS_NBR_WAVES 0             // >0: ask the compiler for at least this many waves per SIMD
 #endif
@@ -33,9 +39,10 @@ constexpr int kChunkSpan = BS_CHUNK_SPAN;   // cells one neighbour-kernel chunk 
 // Everything a kernel needs about the rules, by value in the kernarg segment.
 struct DevParams {
     uint32_t n;
-    int32_t G;                 // cells per axis (grid modes)
+    int32_t G;                 // cells per axis (grid modes); x has sx * G columns
+    int32_t sx;                // x columns per cell edge (grid modes): 1 or 2
     double F, half;            // field edge, F/2
-    double inv_edge;           // G / F
+    double inv_edge;           // G / F (x columns: sx * G / F)
     // double-precision exact thresholds (FP64 modes and mixed rechecks)
     double s_thr[3];           // d < r  <=>  s < s_thr   (s = squared distance, no FMA)
     double c_thr[3];           // acos(c) <= pi*a  <=>  c >= c_thr && c <= 1
@@ -81,12 +88,12 @@ struct Lattice {               // mixed mode companion of the sorted FP64 state
 // One flock of a grid engine.  A grid engine simulates one or more
 // independent flocks (a parameter sweep); flock f owns the boid ids/slots
 // [boid_base, boid_base + P.n) and the global cell keys
-// [cell_base, cell_base + G^3), so one counting sort over global keys keeps
+// [cell_base, cell_base + sx G^3), so one counting sort over global keys keeps
 // every flock contiguous.  (y,z) rows are numbered globally from row_base.
 struct FlockDesc {
     DevParams P;
     uint32_t boid_base, cell_base, row_base, nrows;
-    uint32_t span;             // cells per neighbour-kernel chunk group (<= kSpanMax)
+    uint32_t span;             // x columns per neighbour-kernel chunk group (<= kSpanMax)
     uint32_t pad[3];
 };
 
@@ -121,13 +128,14 @@ hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid,
                          const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
                          uint32_t* key_sorted, Lattice* lat, double2* rec, hipStream_t stream);
 // Neighbour kernels (boids_nbr.hip).  nb = workgroup size (64, 128 or 256),
-// tile = LDS candidates per thread and pass (6, 9 or 12), max_span =
-// largest FlockDesc::span, max_chunks = capacity of `table`.
-hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in, const double2* rec,
+// tile = LDS candidates per thread and pass (6, 9 or 12), max_cols = most x
+// columns of a chunk's tile (columns a chunk covers + 2 sx), max_chunks =
+// capacity of `table`.
+hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_cols, StateF64 in, const double2* rec,
                           StateF64 out, const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                           const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
                           uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream);
-hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_span, StateF64 in, StateF64 out, Lattice lat,
+hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_cols, StateF64 in, StateF64 out, Lattice lat,
                             const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                             const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,

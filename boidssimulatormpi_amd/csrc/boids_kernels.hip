@@ -170,15 +170,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
 }
 
 // Chunk table of the neighbour kernels, entries (local row, first slot, end
-// slot, flock).  Every (y,z) row of cells of every flock is cut into groups
-// of `span` consecutive cells, and every group into runs of `chunk`
-// consecutive sorted boids, cut short where a run would cover more than
-// kChunkSpan cells (sparse stretches), so a chunk lies in one row, holds at
-// most `chunk` boids and its tile tables stay small.  A short cut moves the
-// run's first cell on by at least kChunkSpan, so a group has at most
-// (span - 1) / kChunkSpan of them: the table holds at most n / chunk +
-// groups * (1 + (span - 1) / kChunkSpan) entries.  One wave per row: the
-// row's cell starts are read into LDS (coalesced) and one lane walks them.
+// slot, flock).  Every (y,z) row of cells of every flock (sx G x columns)
+// is cut into groups of `span` consecutive columns, and every group into
+// runs of `chunk` consecutive sorted boids, cut short where a run would cover
+// more than kChunkSpan columns (sparse stretches), so a chunk lies in one
+// row, holds at most `chunk` boids and its tile tables stay small.  A short
+// cut moves the run's first column on by at least kChunkSpan, so a group has
+// at most (span - 1) / kChunkSpan of them: the table holds at most n / chunk
+// + groups * (1 + (span - 1) / kChunkSpan) entries.  One wave per row: the
+// row's column starts are read into LDS (coalesced) and one lane walks them.
 // Pass 1 counts the row's chunks; after a scan of the counts, pass 2 writes
 // them at the row's offset (rows in order: neighbouring rows' chunks stay
 // adjacent in the table).
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
                                                               uint32_t* row_count, const uint32_t* row_off,
                                                               uint4* table, uint32_t* nchunks, uint32_t chunk)
 {
-    __shared__ uint32_t rs[kRowWaves][kMaxCells + 1];
+    __shared__ uint32_t rs[kRowWaves][kSubXMax * kMaxCells + 1];
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kRowWaves + w;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
     }
     const uint32_t f = a;
     const FlockDesc& fd = F.fl[f];
-    const uint32_t G = (uint32_t)fd.P.G, span = fd.span;
+    const uint32_t G = (uint32_t)fd.P.sx * (uint32_t)fd.P.G, span = fd.span;   // x columns of the row
     const uint32_t rl = r - fd.row_base;
     const uint32_t c0 = fd.cell_base + rl * G;
     uint32_t* cs = rs[w];

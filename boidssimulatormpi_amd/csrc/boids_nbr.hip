@@ -3,12 +3,13 @@
 // src/boid_simulation.cpp:153-212 per pair, phase B :216-248).
 //
 // A workgroup of NB threads owns a chunk [t0, t1) of sorted boids inside
-// one (y,z) row of cells, spanning cells cxa..cxb of that row (at most
-// kChunkSpan).  Its tile is the 3 x 3 neighbour rows over the columns
-// cxa-1 .. cxb+1, ordered column-major: entry e = 9 * column + rr, with
-// rr = 3 * (dz + 1) + (dy + 1), each entry the boids of one cell in sorted
-// order.  A boid in column L therefore finds its whole 27-cell window in the
-// one contiguous tile range [off(L-1, 0), off(L+2, 0)).
+// one (y,z) row of cells, spanning x columns cxa..cxb of that row (at most
+// kChunkSpan; a row has sx columns per cell edge).  Its tile is the 3 x 3
+// neighbour rows over the columns cxa-sx .. cxb+sx, ordered column-major:
+// entry e = 9 * column + rr, with rr = 3 * (dz + 1) + (dy + 1), each entry
+// the boids of one (column, row) cell in sorted order.  A boid in column L
+// therefore finds its whole window, columns L-sx .. L+sx of the 9 rows, in
+// the one contiguous tile range [off(L-sx, 0), off(L+sx+1, 0)).
 //
 // Per candidate the tile holds a coarse position (top 16 bits of the 32-bit
 // lattice coordinate per axis) and the index of its tile entry (cell); the
@@ -33,17 +34,18 @@ template <int NB, int TD> struct NbrCfg {
 };
 
 // Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
-// sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg.
-__host__ __device__ constexpr size_t tile_table_bytes(int max_span)
+// sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg
+// for a tile of nseg <= max_cols columns.
+__host__ __device__ constexpr size_t tile_table_bytes(int max_cols)
 {
-    return ((size_t)(9 * (max_span + 2) + 1) * 4 + (size_t)(9 * (max_span + 2)) * 4 + 15) & ~(size_t)15;
+    return ((size_t)(9 * max_cols + 1) * 4 + (size_t)(9 * max_cols) * 4 + 15) & ~(size_t)15;
 }
 
 // Builds the entry tables of the chunk's tile (counts, then a block-wide
 // exclusive scan; each thread owns consecutive entries).
 template <int NB>
 __device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ start, uint32_t cbase, int G,
-                                                  int cxa, int nseg, int cy, int cz, uint32_t* off,
+                                                  int sx, int cxa, int nseg, int cy, int cz, uint32_t* off,
                                                   uint32_t* gst, uint32_t* wsum)
 {
     const int E = 9 * nseg;
@@ -53,12 +55,13 @@ __device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ s
     uint32_t local = 0;
     for (int e = e0; e < e1; ++e) {
         const int xc = e / 9, rr = e - 9 * xc;
-        int cx = cxa - 1 + xc;
-        if (cx < 0) cx += G; else if (cx >= G) cx -= G;
+        const int Gx = sx * G;
+        int cx = cxa - sx + xc;             // in [-sx, Gx - 1 + sx]: one wrap at most
+        if (cx < 0) cx += Gx; else if (cx >= Gx) cx -= Gx;
         int ny = cy + rr % 3 - 1, nz = cz + rr / 3 - 1;
         if (ny < 0) ny += G; else if (ny >= G) ny -= G;
         if (nz < 0) nz += G; else if (nz >= G) nz -= G;
-        const uint32_t c = cbase + (uint32_t)cx + (uint32_t)G * ((uint32_t)ny + (uint32_t)G * (uint32_t)nz);
+        const uint32_t c = cbase + (uint32_t)cx + (uint32_t)Gx * ((uint32_t)ny + (uint32_t)G * (uint32_t)nz);
         const uint32_t a = start[c], b = start[c + 1];
         gst[e] = a;
         off[e] = b - a;                 // the count, until the scan below
@@ -256,10 +259,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const DevParams P = FL.fl[f].P;
     const uint32_t cbase = FL.fl[f].cell_base;
     const int G = P.G;
+    const int sx = P.sx;                              // x columns per cell edge
+    const uint32_t Gx = (uint32_t)(sx * G);           // x columns of a row
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
-    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
-    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
-    const int nseg = cxb - cxa + 3;
+    const int cxa = (int)((key_sorted[t0] - cbase) % Gx);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % Gx);
+    const int nseg = cxb - cxa + 1 + 2 * sx;
     const int E = 9 * nseg;
     uint32_t* off = (uint32_t*)dyn;
     uint32_t* gst = off + (E + 1);
@@ -270,12 +275,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const bool active = s < t1;
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
     uint32_t myid = 0;
-    int L = 1;
+    int L = sx;
     if (active) {
         px = in.x[s]; py = in.y[s]; pz = in.z[s];
         vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
         myid = in.id[s];
-        L = (int)((key_sorted[s] - cbase) % (uint32_t)G) - cxa + 1;
+        L = (int)((key_sorted[s] - cbase) % Gx) - cxa + sx;
     }
     const double sc = P.lat_scale;
     const uint2 mc = coarse_of(to_lattice(px, sc), to_lattice(py, sc), to_lattice(pz, sc));
@@ -292,9 +297,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const double F = P.F, half = P.half;
     const bool k_c = P.cone[COH], k_s = P.cone[SEP], k_a = P.cone[ALI];
 
-    build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
-    const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
+    const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
     uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
 
     // One (owner, candidate) pair per lane: the candidate's record from L2
@@ -468,10 +473,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const DevParams P = FL.fl[f].P;
     const uint32_t cbase = FL.fl[f].cell_base;
     const int G = P.G;
+    const int sx = P.sx;                              // x columns per cell edge
+    const uint32_t Gx = (uint32_t)(sx * G);           // x columns of a row
     const int cy = (int)(row % (uint32_t)G), cz = (int)(row / (uint32_t)G);
-    const int cxa = (int)((key_sorted[t0] - cbase) % (uint32_t)G);
-    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % (uint32_t)G);
-    const int nseg = cxb - cxa + 3;
+    const int cxa = (int)((key_sorted[t0] - cbase) % Gx);
+    const int cxb = (int)((key_sorted[t1 - 1] - cbase) % Gx);
+    const int nseg = cxb - cxa + 1 + 2 * sx;
     const int E = 9 * nseg;
     uint32_t* off = (uint32_t*)dyn;
     uint32_t* gst = off + (E + 1);
@@ -483,13 +490,13 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
     uint32_t myid = 0;
     uint4 me = make_uint4(0, 0, 0, 0);
-    int L = 1;
+    int L = sx;
     if (active) {
         px = in.x[s]; py = in.y[s]; pz = in.z[s];
         vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
         myid = in.id[s];
         me = lat.p[s];
-        L = (int)((key_sorted[s] - cbase) % (uint32_t)G) - cxa + 1;
+        L = (int)((key_sorted[s] - cbase) % Gx) - cxa + sx;
     }
     const uint2 mc = coarse_of(me.x, me.y, me.z);
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
@@ -510,9 +517,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     uint32_t nre = 0;
 
-    build_tile_tables<NB>(start, cbase, G, cxa, nseg, cy, cz, off, gst, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, gst, wsum);
     const uint32_t total = off[E];
-    const uint32_t wlo = off[9 * (L - 1)], whi = off[9 * (L + 2)];
+    const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
     uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
     constexpr double kSub = 1.0 / (double)(1 << kSubBits);
 
@@ -723,15 +730,15 @@ static void launch_mixed_nb(Flocks F, size_t dyn, dim3 grid, StateF64 in, StateF
         return hipErrorInvalidValue;                                          \
     } while (0)
 
-hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in, const double2* rec,
+hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_cols, StateF64 in, const double2* rec,
                           StateF64 out, const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                           const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
                           uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream)
 {
     if (F.n == 0) return hipSuccess;
-    if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
+    if (max_cols < 3 || max_cols > kChunkSpan + 2 * kSubXMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = tile_table_bytes(min(max_span, kChunkSpan));   // a chunk covers <= kChunkSpan cells
+    const size_t dyn = tile_table_bytes(max_cols);
 #define BS_CALL(NB_, TD_) launch_f64_nb<NB_, TD_>(F, dyn, grid, in, rec, out, key_sorted, start, table, nchunks, \
                                                    key_out, rank, count, forces, digest, stream)
     BS_NBR_DISPATCH(nb, tile, BS_CALL);
@@ -739,16 +746,16 @@ hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_span, StateF64 in,
     return hipGetLastError();
 }
 
-hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_span, StateF64 in, StateF64 out, Lattice lat,
+hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_cols, StateF64 in, StateF64 out, Lattice lat,
                             const uint32_t* key_sorted, const uint32_t* start, const uint4* table,
                             const uint32_t* nchunks, uint32_t max_chunks, uint32_t* key_out, uint32_t* rank,
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
                             hipStream_t stream)
 {
     if (F.n == 0) return hipSuccess;
-    if (max_span < 1 || max_span > kSpanMax) return hipErrorInvalidValue;
+    if (max_cols < 3 || max_cols > kChunkSpan + 2 * kSubXMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
-    const size_t dyn = tile_table_bytes(min(max_span, kChunkSpan));   // a chunk covers <= kChunkSpan cells
+    const size_t dyn = tile_table_bytes(max_cols);
 #define BS_CALL(NB_, TD_) launch_mixed_nb<NB_, TD_>(F, dyn, grid, in, out, lat, key_sorted, start, table, nchunks, \
                                                      key_out, rank, count, forces, digest, recheck, stream)
     BS_NBR_DISPATCH(nb, tile, BS_CALL);
