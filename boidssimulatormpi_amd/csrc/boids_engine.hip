@@ -24,6 +24,19 @@
 #endif
 static_assert(BS_MIXED_SUBX >= 1 && BS_MIXED_SUBX <= boids::kSubXMax && BS_F64_SUBX >= 1 &&
                   BS_F64_SUBX <= boids::kSubXMax, "x columns per cell edge");
+// Most x columns a neighbour-kernel chunk covers (k_rows_pass cuts runs of
+// boids short there): narrow chunks give the lanes of a wave windows of more
+// equal length (phase 1 runs as long as the longest), wide ones fewer tile
+// restagings; measured best: 12 half-columns (6 cell edges) for the mixed
+// path, 16 cells for FP64 plain.
+#ifndef BS_MIXED_CHUNK_SPAN
+#define BS_MIXED_CHUNK_SPAN 12
+#endif
+#ifndef BS_F64_CHUNK_SPAN
+#define BS_F64_CHUNK_SPAN 16
+#endif
+static_assert(BS_MIXED_CHUNK_SPAN >= 1 && BS_MIXED_CHUNK_SPAN <= boids::kChunkSpanMax && BS_F64_CHUNK_SPAN >= 1 &&
+                  BS_F64_CHUNK_SPAN <= boids::kChunkSpanMax, "chunk span");
 
 #include <float.h>
 #include <math.h>
@@ -73,6 +86,7 @@ struct bs_engine {
     uint64_t groups = 0;               // (row, span group) pairs over all flocks
     int max_g = 0, max_span = 0;
     int sx = 1;                        // x columns per cell edge of every flock (grid modes)
+    int cspan = kChunkSpanMax;         // most x columns a neighbour-kernel chunk covers
     int nb = 64;                       // neighbour-kernel workgroup size
     int tile = 12;                     // LDS tile depth (candidates per thread and pass)
     int fill = 100;                    // chunk size target, percent of nb
@@ -590,6 +604,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             }
             d.P.G = G;
             d.P.sx = e->sx = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_SUBX : BS_F64_SUBX;
+            e->cspan = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_CHUNK_SPAN : BS_F64_CHUNK_SPAN;
             d.P.inv_edge = G / q.field;
             d.cell_base = (uint32_t)cells;
             d.row_base = (uint32_t)rows;
@@ -600,8 +615,8 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             const uint32_t ng = (Gx + kSpanMax - 1) / kSpanMax;
             d.span = (Gx + ng - 1) / ng;
             // chunk-table slack per span group: its last chunk, plus one chunk
-            // cut short per kChunkSpan columns (k_rows_pass)
-            e->groups += (uint64_t)G * G * ((Gx + d.span - 1) / d.span) * (1 + (d.span - 1) / kChunkSpan);
+            // cut short per cspan columns (k_rows_pass)
+            e->groups += (uint64_t)G * G * ((Gx + d.span - 1) / d.span) * (1 + (d.span - 1) / e->cspan);
             e->max_span = std::max(e->max_span, (int)d.span);
             cells += (uint64_t)Gx * G * G;
             rows += (uint64_t)G * G;
@@ -763,7 +778,7 @@ static int grid_sort_into_a(bs_engine* e)
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
     HIP_TRY(e, launch_rows(flock_view(e), e->start, e->row_count, e->row_off, e->partials, e->table, e->nchunks,
-                           (int)e->chunk, e->stream));
+                           (int)e->chunk, e->cspan, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
     HIP_TRY(e, launch_scatter_ranked(e->prm.n, e->key, e->rank, e->start, e->db.id, e->perm, e->permid, e->stream));
@@ -946,10 +961,10 @@ static int grid_step(bs_engine* e)
     Timed t(e, BS_K_PAIR);
     const Flocks F = flock_view(e);
     if (e->mode == BS_MODE_GRID_F64)
-        HIP_TRY(e, launch_nbr_f64(F, e->nb, e->tile, std::min(e->max_span, kChunkSpan) + 2 * e->sx, e->da, e->rec, e->db, e->key_sorted, e->start, e->table, e->nchunks,
+        HIP_TRY(e, launch_nbr_f64(F, e->nb, e->tile, std::min(e->max_span, e->cspan) + 2 * e->sx, e->da, e->rec, e->db, e->key_sorted, e->start, e->table, e->nchunks,
                                   e->max_chunks, e->key, e->rank, e->count, (double*)e->forces, e->digest, e->stream));
     else
-        HIP_TRY(e, launch_nbr_mixed(F, e->nb, e->tile, std::min(e->max_span, kChunkSpan) + 2 * e->sx, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
+        HIP_TRY(e, launch_nbr_mixed(F, e->nb, e->tile, std::min(e->max_span, e->cspan) + 2 * e->sx, e->da, e->db, e->lat, e->key_sorted, e->start, e->table,
                                     e->nchunks, e->max_chunks, e->key, e->rank, e->count, (double*)e->forces,
                                     e->digest, e->recheck, e->stream));
     t.stop();

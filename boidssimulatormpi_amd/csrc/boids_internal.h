@@ -21,12 +21,9 @@ constexpr int kSubXMax = 2;
 // Neighbour kernels: a workgroup of NB threads (64, 128 or 256) owns a
 // chunk of at most NB sorted boids inside one (y,z) row of cells; rows are
 // cut into span groups of at most kSpanMax x columns, and a chunk covers at
-// most kChunkSpan columns.
+// most the engine's chunk span (<= kChunkSpanMax) columns.
 constexpr int kSpanMax = 62;
-#ifndef BS_CHUNK_SPAN
-#define BS_CHUNK_SPAN 16
-#endif
-constexpr int kChunkSpan = BS_CHUNK_SPAN;   // x columns one neighbour-kernel chunk covers at most
+constexpr int kChunkSpanMax = 16;
 #ifndef BS_NBR_WAVES
 #define BS_NBR_WAVES 0             // >0: ask the compiler for at least this many waves per SIMD
 #endif
@@ -115,8 +112,10 @@ hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* ran
                              hipStream_t stream);
 hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
                        hipStream_t stream);
+// chunk = most boids per chunk, cspan = most x columns a chunk covers.
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, hipStream_t stream);
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan,
+                       hipStream_t stream);
 hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
                           const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
 // Scatter by the arrival ranks the counting pass recorded (no atomics).

@@ -173,11 +173,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
 // slot, flock).  Every (y,z) row of cells of every flock (sx G x columns)
 // is cut into groups of `span` consecutive columns, and every group into
 // runs of `chunk` consecutive sorted boids, cut short where a run would cover
-// more than kChunkSpan columns (sparse stretches), so a chunk lies in one
+// more than `cspan` columns (sparse stretches), so a chunk lies in one
 // row, holds at most `chunk` boids and its tile tables stay small.  A short
-// cut moves the run's first column on by at least kChunkSpan, so a group has
-// at most (span - 1) / kChunkSpan of them: the table holds at most n / chunk
-// + groups * (1 + (span - 1) / kChunkSpan) entries.  One wave per row: the
+// cut moves the run's first column on by at least cspan, so a group has at
+// most (span - 1) / cspan of them: the table holds at most n / chunk +
+// groups * (1 + (span - 1) / cspan) entries.  One wave per row: the
 // row's column starts are read into LDS (coalesced) and one lane walks them.
 // Pass 1 counts the row's chunks; after a scan of the counts, pass 2 writes
 // them at the row's offset (rows in order: neighbouring rows' chunks stay
@@ -187,7 +187,8 @@ constexpr int kRowWaves = 4;
 template <bool WRITE>
 __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const uint32_t* __restrict__ start,
                                                               uint32_t* row_count, const uint32_t* row_off,
-                                                              uint4* table, uint32_t* nchunks, uint32_t chunk)
+                                                              uint4* table, uint32_t* nchunks, uint32_t chunk,
+                                                              uint32_t cspan)
 {
     __shared__ uint32_t rs[kRowWaves][kSubXMax * kMaxCells + 1];
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
         while (t0 < hi) {
             while (cs[xa + 1] <= t0) ++xa;          // cell of boid t0
             uint32_t t1 = min(hi, t0 + chunk);
-            if (xa + kChunkSpan < x1) t1 = min(t1, cs[xa + kChunkSpan]);
+            if (xa + cspan < x1) t1 = min(t1, cs[xa + cspan]);
             if (WRITE) table[k] = make_uint4(rl, t0, t1, f);
             ++k;
             t0 = t1;
@@ -349,16 +350,17 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
 }
 
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, hipStream_t stream)
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan,
+                       hipStream_t stream)
 {
-    if (F.count == 0 || chunk <= 0 || F.rows == 0) return hipErrorInvalidValue;
+    if (F.count == 0 || chunk <= 0 || F.rows == 0 || cspan < 1 || cspan > kChunkSpanMax) return hipErrorInvalidValue;
     const dim3 grid((F.rows + kRowWaves - 1) / kRowWaves);
     hipLaunchKernelGGL(k_rows_pass<false>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk);
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan);
     hipError_t st = launch_scan(row_count, row_off, F.rows + 1, partials, stream);
     if (st != hipSuccess) return st;
     hipLaunchKernelGGL(k_rows_pass<true>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk);
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
 //
 // A workgroup of NB threads owns a chunk [t0, t1) of sorted boids inside
 // one (y,z) row of cells, spanning x columns cxa..cxb of that row (at most
-// kChunkSpan; a row has sx columns per cell edge).  Its tile is the 3 x 3
+// the engine's chunk span; a row has sx columns per cell edge).  Its tile is the 3 x 3
 // neighbour rows over the columns cxa-sx .. cxb+sx, ordered column-major:
 // entry e = 9 * column + rr, with rr = 3 * (dz + 1) + (dy + 1), each entry
 // the boids of one (column, row) cell in sorted order.  A boid in column L
@@ -736,7 +736,7 @@ hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_cols, StateF64 in,
                           uint32_t* count, double* forces, uint64_t* digest, hipStream_t stream)
 {
     if (F.n == 0) return hipSuccess;
-    if (max_cols < 3 || max_cols > kChunkSpan + 2 * kSubXMax) return hipErrorInvalidValue;
+    if (max_cols < 3 || max_cols > kChunkSpanMax + 2 * kSubXMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
     const size_t dyn = tile_table_bytes(max_cols);
 #define BS_CALL(NB_, TD_) launch_f64_nb<NB_, TD_>(F, dyn, grid, in, rec, out, key_sorted, start, table, nchunks, \
@@ -753,7 +753,7 @@ hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_cols, StateF64 i
                             hipStream_t stream)
 {
     if (F.n == 0) return hipSuccess;
-    if (max_cols < 3 || max_cols > kChunkSpan + 2 * kSubXMax) return hipErrorInvalidValue;
+    if (max_cols < 3 || max_cols > kChunkSpanMax + 2 * kSubXMax) return hipErrorInvalidValue;
     const dim3 grid(round8(max_chunks));
     const size_t dyn = tile_table_bytes(max_cols);
 #define BS_CALL(NB_, TD_) launch_mixed_nb<NB_, TD_>(F, dyn, grid, in, out, lat, key_sorted, start, table, nchunks, \
