@@ -12,28 +12,27 @@
 #ifndef BS_F64_TILE
 #define BS_F64_TILE 6
 #endif
-// x columns per cell edge of the grid modes (DevParams::sx): the mixed path
-// bins x at half the cell edge (a 2.5-edge window along x instead of 3);
-// the FP64 path keeps the reference's cells, where the shorter chunks that
-// finer columns bring cost more than the smaller windows save.
+// x columns per cell edge of the grid modes (DevParams::sx): both paths bin
+// x at half the cell edge, a 2.5-edge window along x instead of 3 (with
+// 12-column chunks; with 16-column chunks the FP64 kernel was faster on the
+// reference's cells).
 #ifndef BS_MIXED_SUBX
 #define BS_MIXED_SUBX 2
 #endif
 #ifndef BS_F64_SUBX
-#define BS_F64_SUBX 1
+#define BS_F64_SUBX 2
 #endif
 static_assert(BS_MIXED_SUBX >= 1 && BS_MIXED_SUBX <= boids::kSubXMax && BS_F64_SUBX >= 1 &&
                   BS_F64_SUBX <= boids::kSubXMax, "x columns per cell edge");
 // Most x columns a neighbour-kernel chunk covers (k_rows_pass cuts runs of
 // boids short there): narrow chunks give the lanes of a wave windows of more
 // equal length (phase 1 runs as long as the longest), wide ones fewer tile
-// restagings; measured best: 12 half-columns (6 cell edges) for the mixed
-// path, 16 cells for FP64 plain.
+// restagings; measured best for both paths: 12 half-columns (6 cell edges).
 #ifndef BS_MIXED_CHUNK_SPAN
 #define BS_MIXED_CHUNK_SPAN 12
 #endif
 #ifndef BS_F64_CHUNK_SPAN
-#define BS_F64_CHUNK_SPAN 16
+#define BS_F64_CHUNK_SPAN 12
 #endif
 static_assert(BS_MIXED_CHUNK_SPAN >= 1 && BS_MIXED_CHUNK_SPAN <= boids::kChunkSpanMax && BS_F64_CHUNK_SPAN >= 1 &&
                   BS_F64_CHUNK_SPAN <= boids::kChunkSpanMax, "chunk span");

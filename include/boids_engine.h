@@ -42,13 +42,15 @@ enum {
     BS_MODE_ALLPAIRS_F32REF = 0,
     /* the same algorithm promoted to double (all-pairs). */
     BS_MODE_ALLPAIRS_F64 = 1,
-    /* double, uniform grid + 27-cell window (boidsim_grid's role,
-     * src/boid_simulation_grid.cpp:165-293).  Falls back to all-pairs when
-     * the field holds fewer than 3 cells per axis. */
+    /* double, uniform grid of cells with edge >= the largest radius (x
+     * binned at half the edge), every cell within reach searched
+     * (boidsim_grid's role, src/boid_simulation_grid.cpp:165-293).  Falls
+     * back to all-pairs when the field holds fewer than 3 cells per axis. */
     BS_MODE_GRID_F64 = 2,
     /* mixed precision grid: FP64 master state and position update, FP32
-     * pair tests and Kahan-compensated FP32 force sums, FP64 recheck of
-     * pairs inside a guard band around every threshold. */
+     * pair tests on exact 32-bit lattice differences, FP64 recheck of pairs
+     * inside a guard band around every threshold, force sums of
+     * integer-valued terms in FP64 (exact, so independent of order). */
     BS_MODE_GRID_MIXED = 3
 };
 
