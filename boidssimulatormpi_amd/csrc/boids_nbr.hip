@@ -34,7 +34,8 @@ template <int NB, int TD> struct NbrCfg {
 };
 
 // Entry tables of a tile in dynamic LDS: off[E + 1] (prefix of the entry
-// sizes) and gst[E] (global sorted start of each entry's cell), E = 9 * nseg
+// sizes) and sbase[E] (slot of the entry's first boid minus off[e], so a tile
+// position p holds slot sbase[e] + p), E = 9 * nseg
 // for a tile of nseg <= max_cols columns.
 __host__ __device__ constexpr size_t tile_table_bytes(int max_cols)
 {
@@ -46,7 +47,7 @@ __host__ __device__ constexpr size_t tile_table_bytes(int max_cols)
 template <int NB>
 __device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ start, uint32_t cbase, int G,
                                                   int sx, int cxa, int nseg, int cy, int cz, uint32_t* off,
-                                                  uint32_t* gst, uint32_t* wsum)
+                                                  uint32_t* sbase, uint32_t* wsum)
 {
     const int E = 9 * nseg;
     const int per = (E + NB - 1) / NB;
@@ -63,7 +64,7 @@ __device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ s
         if (nz < 0) nz += G; else if (nz >= G) nz -= G;
         const uint32_t c = cbase + (uint32_t)cx + (uint32_t)Gx * ((uint32_t)ny + (uint32_t)G * (uint32_t)nz);
         const uint32_t a = start[c], b = start[c + 1];
-        gst[e] = a;
+        sbase[e] = a;                     // the cell's sorted start, until the scan below
         off[e] = b - a;                 // the count, until the scan below
         local += b - a;
     }
@@ -72,6 +73,7 @@ __device__ __forceinline__ void build_tile_tables(const uint32_t* __restrict__ s
     for (int e = e0; e < e1; ++e) {
         const uint32_t cnt = off[e];
         off[e] = run;
+        sbase[e] -= run;                  // slot = sbase[e] + tile position (mod 2^32)
         run += cnt;
     }
     if (threadIdx.x == 0) off[E] = total;
@@ -96,7 +98,7 @@ __device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, v2s my_z)
 // Stages tile positions [base, base + here); each thread copies one run of
 // consecutive positions, calling STORE(position - base, slot, entry).
 template <int NB, typename STORE>
-__device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* gst, int E, uint32_t base,
+__device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* sbase, int E, uint32_t base,
                                            uint32_t here, STORE store)
 {
     const uint32_t per = (here + NB - 1) / NB;
@@ -109,10 +111,10 @@ __device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* 
         if (off[mid] <= p0) lo = mid; else hi = mid - 1;
     }
     int e = lo;
-    uint32_t eb = off[e], ee = off[e + 1], g = gst[e];
+    uint32_t ee = off[e + 1], g = sbase[e];
     for (uint32_t p = p0; p < p1; ++p) {
-        while (p >= ee) { ++e; eb = ee; ee = off[e + 1]; g = gst[e]; }
-        store(p - base, g + (p - eb), (uint32_t)e);
+        while (p >= ee) { ++e; ee = off[e + 1]; g = sbase[e]; }
+        store(p - base, g + p, (uint32_t)e);
     }
 }
 
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const int nseg = cxb - cxa + 1 + 2 * sx;
     const int E = 9 * nseg;
     uint32_t* off = (uint32_t*)dyn;
-    uint32_t* gst = off + (E + 1);
+    uint32_t* sbase = off + (E + 1);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wbase = tid - lane;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const double F = P.F, half = P.half;
     const bool k_c = P.cone[COH], k_s = P.cone[SEP], k_a = P.cone[ALI];
 
-    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, gst, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
     uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         r.e = list[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
         const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
-        r.j = gst[e] + (base + ((r.e >> 6) & 0xFFFFu) - off[e]);
+        r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
         r.a = rec[3 * (size_t)r.j];
         r.b = rec[3 * (size_t)r.j + 1];
         r.c = rec[3 * (size_t)r.j + 2];
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
         const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
         __syncthreads();
-        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
+        stage_tile<NB>(off, sbase, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
             const uint2 c = coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
             tco[kk] = make_uint2(c.x, c.y | (e << 22));
         });
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const int nseg = cxb - cxa + 1 + 2 * sx;
     const int E = 9 * nseg;
     uint32_t* off = (uint32_t*)dyn;
-    uint32_t* gst = off + (E + 1);
+    uint32_t* sbase = off + (E + 1);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wbase = tid - lane;
@@ -517,7 +519,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     uint32_t nre = 0;
 
-    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, gst, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, wsum);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
     uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
@@ -574,7 +576,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         r.e = lst[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
         const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
-        r.j = gst[e] + (base + ((r.e >> 6) & 0xFFFFu) - off[e]);
+        r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
         r.q = lat.p[r.j];
         r.w = lat.v[r.j];
         return r;
@@ -630,7 +632,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     for (uint32_t base = 0; base < total; base += Cfg::kCap) {
         const uint32_t here = min((uint32_t)Cfg::kCap, total - base);
         __syncthreads();
-        stage_tile<NB>(off, gst, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
+        stage_tile<NB>(off, sbase, E, base, here, [&](uint32_t kk, uint32_t slot, uint32_t e) {
             const uint4 q = lat.p[slot];
             const uint2 c = coarse_of(q.x, q.y, q.z);
             tco[kk] = make_uint2(c.x, c.y | (e << 22));
