@@ -317,10 +317,10 @@ static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
         if (isnan(r[k])) { P->s_lo[k] = 0.0f; P->s_hi[k] = 0.0f; }
         P->s_hi_max = std::max(P->s_hi_max, P->s_hi[k]);
         const double cth = P->c_thr[k];
-        P->c_lo[k] = round_down_f(cth - beta_c);
-        P->c_hi[k] = round_up_f(cth + beta_c);
+        P->c_lo[k] = P->cone[k] ? round_down_f(cth - beta_c) : -INFINITY;
+        P->c_hi[k] = P->cone[k] ? round_up_f(cth + beta_c) : -INFINITY;
+        P->c_top[k] = P->cone[k] ? round_down_f(1.0 - beta_c) : INFINITY;
     }
-    P->c_top = round_down_f(1.0 - beta_c);
     P->band_k = 8.0f;   // >= 2*sqrt(3) lattice units of displacement error, x2 margin
     // Coarse filter on the top 16 lattice bits (unit F/2^16): a coarse
     // difference is off by at most one unit per axis, so every pair with

@@ -59,7 +59,8 @@ struct DevParams {
     float s_lo[3], s_hi[3];    // squared lattice distance: < s_lo surely in, >= s_hi surely out
     float s_hi_max;
     float c_lo[3], c_hi[3];    // c32 <= c_lo - k/d surely out, c32 >= c_hi + k/d surely in
-    float c_top;               // c32 >= c_top: too close to 1, recheck
+                               // (a rule without view cone: -inf, the cosine decides nothing)
+    float c_top[3];            // c32 >= c_top: too close to 1, recheck (no view cone: +inf)
     float band_k;              // k: cosine band growth with 1/d (lattice units)
     int32_t coarse_thr;        // squared distance bound on the top 16 lattice bits
     // velocity quantum of the mixed lattice records: a power of two with

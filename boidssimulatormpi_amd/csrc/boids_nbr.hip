@@ -419,16 +419,16 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
 // ---------------------------------------------------------------------------
 // FP32 verdict of one rule: in = surely inside, out = surely outside
 // (neither: inside a guard band, decided in FP64).  c_dn / c_up are the
-// FP32 cosine moved down / up by the displacement band; c_ok = cosine not
-// within 1e-5 of 1.  A skipped candidate is surely outside every rule.
-__device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, float c_dn, float c_up, bool c_ok,
+// FP32 cosine c moved down / up by the displacement band; c < c_top keeps
+// it off 1.  A rule without a view cone has c_lo = c_hi = -inf and c_top =
+// +inf, so the cosine decides nothing there (a NaN cosine falls to the FP64
+// decision).  Only compares and their lane masks: no per-lane selects.  A
+// skipped candidate is surely outside every rule.
+__device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, float c, float c_dn, float c_up,
                                            bool skip, bool& in, bool& out)
 {
-    const bool cone = P.cone[r];
-    const bool s_in = s2 < P.s_lo[r];
-    const bool s_out = !(s2 < P.s_hi[r]);
-    in = s_in & (!cone | ((c_dn >= P.c_hi[r]) & c_ok)) & !skip;
-    out = s_out | (cone & (c_up <= P.c_lo[r])) | skip;
+    in = (s2 < P.s_lo[r]) & (c_dn >= P.c_hi[r]) & (c < P.c_top[r]) & !skip;
+    out = !(s2 < P.s_hi[r]) | (c_up <= P.c_lo[r]) | skip;
 }
 
 constexpr double kSepQ = 1073741824.0;                // separation terms in quanta of 2^-30
@@ -594,15 +594,13 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const float c = fmaf(r.ov.z, gz, fmaf(r.ov.y, gy, r.ov.x * gx)) * inv_d * r.ov.w;
         const float band = P.band_k * inv_d;
         const float c_dn = c - band, c_up = c + band;
-        const bool c_ok = c < P.c_top;
-        const bool skip = !r.valid || r.j == t0 + r.o;    // padding, or the owner itself
+        const bool skip = !r.valid | (r.j == t0 + r.o);    // padding, or the owner itself
         bool in_c, out_c, in_s, out_s, in_a, out_a;
-        rule_bands(P, COH, s2, c_dn, c_up, c_ok, skip, in_c, out_c);
-        rule_bands(P, SEP, s2, c_dn, c_up, c_ok, skip, in_s, out_s);
-        rule_bands(P, ALI, s2, c_dn, c_up, c_ok, skip, in_a, out_a);
+        rule_bands(P, COH, s2, c, c_dn, c_up, skip, in_c, out_c);
+        rule_bands(P, SEP, s2, c, c_dn, c_up, skip, in_s, out_s);
+        rule_bands(P, ALI, s2, c, c_dn, c_up, skip, in_a, out_a);
         const bool obad = !(r.ov.w > 0.f);
-        const bool sure = (in_c || out_c) && (in_s || out_s) && (in_a || out_a) &&
-                          !(obad && !(out_c && out_s && out_a));
+        const bool sure = (in_c | out_c) & (in_s | out_s) & (in_a | out_a) & !(obad & !(out_c & out_s & out_a));
         if (!sure) {
             const uint32_t so = t0 + r.o;
             const double ovx = in.vx[so], ovy = in.vy[so], ovz = in.vz[so];
