@@ -343,6 +343,7 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
 {
     const uint32_t nt = (m + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, stream, count, start, m, partials);
+    if (nt <= 1) return hipGetLastError();      // one tile: its local scan is the scan
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, stream, partials, nt);
     hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanThreads), 0, stream, start, m,
                        (const uint32_t*)partials);
