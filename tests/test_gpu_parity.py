@@ -152,6 +152,34 @@ def test_grid_neighbour_sets_each_step(mode, n, F, rules):
     say(f"per-step max scaled error pos={worst[0]:.3e} vel={worst[1]:.3e} rechecks={e.rechecks()}")
 
 
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+@pytest.mark.parametrize("F,cells", [(0.25, 3), (0.33, 4)])
+def test_smallest_grids_wrapped_tiles(mode, F, cells):
+    # 3 or 4 cells per axis, x binned at half the cell edge: 6 or 8 x columns,
+    # so a chunk's tile (its columns plus two on either side) wraps around the
+    # row and can hold a column twice, while each boid's 5-column window
+    # (distinct columns) must still see every neighbour exactly once.  Dense:
+    # ~400 cohesion neighbours per boid.
+    n = 3000
+    st = iso_state(n, F, 5)
+    e = make(n, F, CFG0, mode, E.FLAG_DIGEST)
+    assert e.info().mode == mode and e.info().cells_per_axis == cells
+    e.set_state(xyzv6=st)
+    P = O.params(F, **CFG0)
+    gp, gv = e.state_f64()
+    for t in range(4):
+        p, v = gp.copy(), gv.copy()
+        _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+        e.step(1)
+        bad = np.nonzero((e.digest() != d).any(1))[0]
+        assert bad.size == 0, f"step {t}: {bad.size} boids with different neighbour sets, e.g. {bad[:5]}"
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
+        tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+        assert ep <= tol and ev <= tol, (t, ep, ev)
+    say(f"{cells} cells per axis {E.MODE_NAMES[mode]} ok, rechecks={e.rechecks()}")
+
+
 def test_grid_f64_ten_steps_continuous():
     n, F = 16384, 1.0
     st = iso_state(n, F, 1)
