@@ -192,7 +192,7 @@ __device__ __forceinline__ void finish_boid(const DevParams& P, uint32_t cbase, 
 }
 
 // ---------------------------------------------------------------------------
-// FP64 plain 27-cell kernel.
+// Neighbour kernels: chunk scheduling.
 // ---------------------------------------------------------------------------
 // Chunk of workgroup b, or nchunks (no work) for the grid's spare
 // workgroups.  Consecutive workgroups land on different XCDs (8 on

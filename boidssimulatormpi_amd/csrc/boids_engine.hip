@@ -598,7 +598,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             if (G < 0) { release(e); return fail(nullptr, BS_E_ARG, at + "cells_per_axis must be >= 3 with edge >= max radius"); }
             if (G < 3) {
                 if (batch) { release(e); return fail(nullptr, BS_E_ARG, at + "fewer than 3 cells per axis (field too small for its radii); step it in its own engine"); }
-                e->mode = BS_MODE_ALLPAIRS_F64;   // 27-cell windows would overlap
+                e->mode = BS_MODE_ALLPAIRS_F64;   // windows (3 cells in y and z) would overlap
                 break;
             }
             d.P.G = G;
