@@ -232,10 +232,10 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     using LC = ListCfg<BS_F64_ROUNDS>;
     constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
     constexpr int NW = NB / 64;
+    // The tile first, at LDS address 0 (the largest variable is placed
+    // first), so phase 1's reads need no base-address additions.  tco:
     // coarse positions, tile entry in bits 22-31 of .y; + 4: a finished
-    // lane's look-ahead reads (positions stay <= base + here)
-    // the tile first, at LDS address 0 (the largest variable is placed
-    // first), so phase 1's reads need no base-address additions
+    // lane's look-ahead reads (positions stay <= base + here).
     __shared__ struct {
         uint2 tco[Cfg::kCap + 4];
         double acc[9][NB];                      // sums of q (cohesion), e/|e| (separation), v (alignment)
@@ -448,10 +448,10 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     using LC = ListCfg<BS_MIXED_ROUNDS>;
     constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
     constexpr int NW = NB / 64;
+    // The tile first, at LDS address 0 (the largest variable is placed
+    // first), so phase 1's reads need no base-address additions.  tco:
     // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
-    // + 4: a finished lane's look-ahead reads
-    // the tile first, at LDS address 0 (the largest variable is placed
-    // first), so phase 1's reads need no base-address additions
+    // + 4: a finished lane's look-ahead reads.
     __shared__ struct {
         uint2 tco[Cfg::kCap + 4];
         double acc[9][NB];                      // integer-valued sums: cohesion, separation, alignment
