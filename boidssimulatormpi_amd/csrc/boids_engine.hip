@@ -81,6 +81,7 @@ struct bs_engine {
     std::vector<double> flock_field;
     FlockDesc* flocks_dev = nullptr;
     uint32_t* flock_of = nullptr;      // slot -> flock (batches only)
+    uint32_t* row_flock = nullptr;     // (y,z) row -> flock (batches only)
     uint32_t rows = 0;
     uint64_t groups = 0;               // (row, span group) pairs over all flocks
     int max_g = 0, max_span = 0;
@@ -530,6 +531,7 @@ static Flocks flock_view(const bs_engine* e)
     Flocks F;
     F.fl = e->flocks_dev;
     F.flock_of = e->flock_of;
+    F.row_flock = e->row_flock;
     F.count = (uint32_t)e->flocks.size();
     F.n = e->prm.n;
     F.rows = e->rows;
@@ -686,6 +688,12 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             for (uint32_t k = 0; k < e->flocks.size(); ++k)
                 std::fill(fo.begin() + e->flocks[k].boid_base, fo.begin() + e->flocks[k].boid_base + e->flocks[k].P.n, k);
             st = hipMemcpy(e->flock_of, fo.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+            if (st != hipSuccess) { hip_fail(e, st, "hipMemcpy"); return bail(BS_E_HIP); }
+            if ((rc = dev_alloc(e, &e->row_flock, e->rows))) return bail(rc);
+            std::vector<uint32_t> rf(e->rows);
+            for (uint32_t k = 0; k < e->flocks.size(); ++k)
+                std::fill(rf.begin() + e->flocks[k].row_base, rf.begin() + e->flocks[k].row_base + e->flocks[k].nrows, k);
+            st = hipMemcpy(e->row_flock, rf.data(), sizeof(uint32_t) * e->rows, hipMemcpyHostToDevice);
             if (st != hipSuccess) { hip_fail(e, st, "hipMemcpy"); return bail(BS_E_HIP); }
         }
         st = hipMemcpy(e->flocks_dev, e->flocks.data(), sizeof(FlockDesc) * e->flocks.size(), hipMemcpyHostToDevice);

@@ -99,6 +99,7 @@ struct FlockDesc {
 struct Flocks {
     const FlockDesc* fl;       // [count] in device memory
     const uint32_t* flock_of;  // flock of each slot (nullptr: one flock)
+    const uint32_t* row_flock; // flock of each (y,z) row (nullptr: one flock)
     uint32_t count;            // number of flocks
     uint32_t n;                // boids over all flocks
     uint32_t rows;             // (y,z) rows over all flocks

@@ -198,12 +198,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
         else row_count[F.rows] = 0u;            // the scan's total lands in row_off[rows]
     }
     if (r >= F.rows) return;                    // whole waves only: no barrier follows
-    uint32_t a = 0, z = F.count - 1;            // last flock with row_base <= r
-    while (a < z) {
-        const uint32_t m = (a + z + 1) >> 1;
-        if (F.fl[m].row_base <= r) a = m; else z = m - 1;
-    }
-    const uint32_t f = a;
+    const uint32_t f = F.row_flock ? F.row_flock[r] : 0u;
     const FlockDesc& fd = F.fl[f];
     const uint32_t G = (uint32_t)fd.P.sx * (uint32_t)fd.P.G, span = fd.span;   // x columns of the row
     const uint32_t rl = r - fd.row_base;
