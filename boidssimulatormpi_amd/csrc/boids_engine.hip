@@ -589,6 +589,9 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         }
         uint64_t cells = 0, rows = 0;
         bool coarse = false;
+        // grid geometry of the path (every flock of an engine shares the mode)
+        e->sx = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_SUBX : BS_F64_SUBX;
+        e->cspan = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_CHUNK_SPAN : BS_F64_CHUNK_SPAN;
         for (uint32_t k = 0; k < count; ++k) {
             const bs_params& q = fp[k];
             const std::string at = batch ? "flock " + std::to_string(k) + ": " : std::string();
@@ -604,8 +607,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
                 break;
             }
             d.P.G = G;
-            d.P.sx = e->sx = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_SUBX : BS_F64_SUBX;
-            e->cspan = p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_CHUNK_SPAN : BS_F64_CHUNK_SPAN;
+            d.P.sx = e->sx;
             d.P.inv_edge = G / q.field;
             d.cell_base = (uint32_t)cells;
             d.row_base = (uint32_t)rows;
