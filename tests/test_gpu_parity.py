@@ -180,6 +180,50 @@ def test_smallest_grids_wrapped_tiles(mode, F, cells):
     say(f"{cells} cells per axis {E.MODE_NAMES[mode]} ok, rechecks={e.rechecks()}")
 
 
+NO_CONE = dict(sep=(0.02, 1.0, 0.002), ali=(0.05, 1.5, 0.06), coh=(0.08, 2.0, 0.008), vmax=0.005, vmin=0.001)
+
+
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+@pytest.mark.parametrize("name", ["ordered", "no_cone"])
+def test_rules_without_view_cone(mode, name):
+    # a >= 1 skips the view cone (src/boid.cpp:29): the mixed kernel then
+    # decides on distance alone (its cosine bounds are -inf / +inf), and a
+    # NaN cosine (coincident points, zero velocity) must not change that.
+    rules = {"ordered": ORDERED, "no_cone": NO_CONE}[name]
+    n, F = 16384, 1.0
+    P = O.params(F, **rules)
+    e = make(n, F, rules, mode, E.FLAG_DIGEST)
+    e.set_state(xyzv6=iso_state(n, F, 9, rules))
+    gp, gv = e.state_f64()
+    for t in range(4):
+        p, v = gp.copy(), gv.copy()
+        _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+        e.step(1)
+        bad = np.nonzero((e.digest() != d).any(1))[0]
+        assert bad.size == 0, f"step {t}: {bad.size} boids with different neighbour sets, e.g. {bad[:5]}"
+        gp, gv = e.state_f64()
+        ep, ev = scaled_err(gp, gv, p, v, F, rules["vmax"])
+        tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+        assert ep <= tol and ev <= tol, (t, ep, ev)
+    # a coincident pair and a boid at rest.  Without a view cone the pair
+    # counts as separation neighbours, whose term (p_i - p_j) / |p_i - p_j|
+    # is 0/0: the reference's velocity turns NaN there, and so must ours.
+    st = np.array([[0.5, 0.5, 0.5, 0.003, 0, 0], [0.5, 0.5, 0.5, 0.003, 0, 0],
+                   [0.51, 0.5, 0.5, 0.0, 0.0, 0.0], [0.995, 0.5, 0.5, 0.002, 0.001, 0]], np.float32)
+    e = make(4, F, rules, mode, E.FLAG_DIGEST)
+    e.set_state(xyzv6=st)
+    e.step(1)
+    p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    _, d = O.step(P, p, v, digest=True)
+    assert np.array_equal(e.digest(), d)
+    gp, gv = e.state_f64()
+    assert np.array_equal(np.isnan(gp), np.isnan(p)) and np.array_equal(np.isnan(gv), np.isnan(v))
+    ok = ~(np.isnan(p).any(1) | np.isnan(v).any(1))
+    ep, ev = scaled_err(gp[ok], gv[ok], p[ok], v[ok], F, rules["vmax"])
+    assert ep < 1e-9 and ev < 1e-9
+    say(f"{name} {E.MODE_NAMES[mode]} ok")
+
+
 def test_grid_f64_ten_steps_continuous():
     n, F = 16384, 1.0
     st = iso_state(n, F, 1)
