@@ -787,7 +787,7 @@ static int grid_sort_into_a(bs_engine* e)
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
     HIP_TRY(e, launch_rows(flock_view(e), e->start, e->row_count, e->row_off, e->partials, e->table, e->nchunks,
-                           (int)e->chunk, e->cspan, e->stream));
+                           (int)e->chunk, e->cspan, e->sx * e->max_g, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
     HIP_TRY(e, launch_scatter_ranked(e->prm.n, e->key, e->rank, e->start, e->db.id, e->perm, e->permid, e->stream));

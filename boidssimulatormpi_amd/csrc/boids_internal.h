@@ -114,9 +114,10 @@ hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* ran
                              hipStream_t stream);
 hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
                        hipStream_t stream);
-// chunk = most boids per chunk, cspan = most x columns a chunk covers.
+// chunk = most boids per chunk, cspan = most x columns a chunk covers,
+// max_cols = x columns of the longest row (sx * G over the flocks).
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan,
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan, int max_cols,
                        hipStream_t stream);
 hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
                           const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);

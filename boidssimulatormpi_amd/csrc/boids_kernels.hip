@@ -188,9 +188,9 @@ template <bool WRITE>
 __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const uint32_t* __restrict__ start,
                                                               uint32_t* row_count, const uint32_t* row_off,
                                                               uint4* table, uint32_t* nchunks, uint32_t chunk,
-                                                              uint32_t cspan)
+                                                              uint32_t cspan, uint32_t stride)
 {
-    __shared__ uint32_t rs[kRowWaves][kSubXMax * kMaxCells + 1];
+    extern __shared__ uint32_t rs[];            // kRowWaves x stride (>= the longest row's columns + 1)
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kRowWaves + w;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
     const uint32_t G = (uint32_t)fd.P.sx * (uint32_t)fd.P.G, span = fd.span;   // x columns of the row
     const uint32_t rl = r - fd.row_base;
     const uint32_t c0 = fd.cell_base + rl * G;
-    uint32_t* cs = rs[w];
+    uint32_t* cs = rs + w * stride;
     for (uint32_t x = lane; x <= G; x += 64) cs[x] = start[c0 + x];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -346,17 +346,21 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
 }
 
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan,
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan, int max_cols,
                        hipStream_t stream)
 {
-    if (F.count == 0 || chunk <= 0 || F.rows == 0 || cspan < 1 || cspan > kChunkSpanMax) return hipErrorInvalidValue;
+    if (F.count == 0 || chunk <= 0 || F.rows == 0 || cspan < 1 || cspan > kChunkSpanMax || max_cols < 1 ||
+        max_cols > kSubXMax * kMaxCells)
+        return hipErrorInvalidValue;
     const dim3 grid((F.rows + kRowWaves - 1) / kRowWaves);
-    hipLaunchKernelGGL(k_rows_pass<false>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan);
+    const uint32_t stride = (uint32_t)max_cols + 1;
+    const size_t dyn = sizeof(uint32_t) * kRowWaves * stride;
+    hipLaunchKernelGGL(k_rows_pass<false>, grid, dim3(64 * kRowWaves), dyn, stream, F, start, row_count,
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride);
     hipError_t st = launch_scan(row_count, row_off, F.rows + 1, partials, stream);
     if (st != hipSuccess) return st;
-    hipLaunchKernelGGL(k_rows_pass<true>, grid, dim3(64 * kRowWaves), 0, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan);
+    hipLaunchKernelGGL(k_rows_pass<true>, grid, dim3(64 * kRowWaves), dyn, stream, F, start, row_count,
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride);
     return hipGetLastError();
 }
 
