@@ -127,6 +127,12 @@ __device__ __forceinline__ float lane_read(float v, uint32_t src)
 {
     return __builtin_bit_cast(float, lane_read(__builtin_bit_cast(uint32_t, v), src));
 }
+__device__ __forceinline__ double lane_read(double v, uint32_t src)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = lane_read((uint32_t)u, src), hi = lane_read((uint32_t)(u >> 32), src);
+    return __builtin_bit_cast(double, (hi << 32) | lo);
+}
 
 // Mask of the wave's lanes where c holds, taken from the compare's own lane
 // mask (__ballot and __any take an int, which costs a select and a second
@@ -243,8 +249,6 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     auto& tco = sh.tco;
     auto& acc = sh.acc;
     __shared__ uint32_t wl[NW][kListCap];
-    __shared__ double2 own_a[NB], own_b[NB], own_c[NB];   // owner (x, y), (z, vx), (vy, vz)
-    __shared__ double own_n[NB];                // owner |v|
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
@@ -288,10 +292,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const uint2 mc = coarse_of(to_lattice(px, sc), to_lattice(py, sc), to_lattice(pz, sc));
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const v2s my_z = __builtin_bit_cast(v2s, mc.y);
-    own_a[tid] = make_double2(px, py);
-    own_b[tid] = make_double2(pz, vx);
-    own_c[tid] = make_double2(vy, vz);
-    own_n[tid] = norm3(vx, vy, vz);
+    const double vn = norm3(vx, vy, vz);            // owner data stays in registers (read by ds_bpermute)
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
     acc_n[tid] = 0;
@@ -322,10 +323,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     // The oracle's decisions on the minimum image (src/boid.cpp:27-42,
     // src/boid_simulation.cpp:153-212), the pair added to its owner's sums.
     auto eval = [&](const Rec& r) {
-        const uint32_t o = r.o;
-        const double2 oa = own_a[o], ob = own_b[o], oc = own_c[o];
-        const double on = own_n[o];
-        const double ox = oa.x, oy = oa.y, oz = ob.x, ovx = ob.y, ovy = oc.x, ovz = oc.y;
+        // every lane of the wave is active here: each reads its owner's
+        // registers (the owner is a lane of the same wave)
+        const uint32_t o = r.o, ol = r.e & 63u;
+        const double ox = lane_read(px, ol), oy = lane_read(py, ol), oz = lane_read(pz, ol);
+        const double ovx = lane_read(vx, ol), ovy = lane_read(vy, ol), ovz = lane_read(vz, ol);
+        const double on = lane_read(vn, ol);
         const double qx = min_image(r.a.x, ox, F, half);
         const double qy = min_image(r.a.y, oy, F, half);
         const double qz = min_image(r.b.x, oz, F, half);
