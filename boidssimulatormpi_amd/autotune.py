@@ -41,8 +41,18 @@ def auto_cells(p: E.Params) -> int:
     g = math.floor(p.field / (rmax * 1.0001))
     if g < 3:
         return 0
-    cap = max(3.0, math.floor(max(p.n, 27) ** (1.0 / 3.0)))
+    cap = max(3, icbrt(max(p.n, 27)))
     return int(min(g, cap, 1024))
+
+
+def icbrt(n: int) -> int:
+    """Largest g with g**3 <= n (exact for perfect cubes)."""
+    g = int(round(n ** (1.0 / 3.0)))
+    while g ** 3 > n:
+        g -= 1
+    while (g + 1) ** 3 <= n:
+        g += 1
+    return g
 
 
 def _with(p: E.Params, **kw) -> E.Params:

@@ -350,6 +350,16 @@ static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
     return BS_OK;
 }
 
+// Largest g with g^3 <= n (exact; floor(cbrt(n)) can land one below a
+// perfect cube).
+static uint32_t icbrt(uint32_t n)
+{
+    uint64_t g = (uint64_t)cbrt((double)n);
+    while (g * g * g > n) --g;
+    while ((g + 1) * (g + 1) * (g + 1) <= n) ++g;
+    return (uint32_t)g;
+}
+
 static int cells_per_axis(const bs_params* p)
 {
     if (p->cells_per_axis > 0) {
@@ -363,7 +373,7 @@ static int cells_per_axis(const bs_params* p)
     double g = floor(p->field / (rmax * 1.0001));
     if (g < 3.0) return 0;
     // no more cells than boids: finer grids only add empty-cell overhead
-    double cap = std::max(3.0, floor(cbrt((double)std::max<uint32_t>(p->n, 27))));
+    double cap = std::max(3.0, (double)icbrt(std::max<uint32_t>(p->n, 27)));
     g = std::min(g, std::max(cap, 3.0));
     g = std::min(g, (double)kMaxCells);
     return (int)g;

@@ -63,12 +63,15 @@ class PtclFile:
         self.correct_file_size = HEADER_SIZE + self.frame_bytes * self.valid_steps
 
     def check_metadata(self):
+        """Raises ValueError when the header's frame count disagrees with the
+        frames the file holds, or when a partial frame trails the data."""
         if self.valid_steps != self.header.steps:
-            raise ValueError(f"Header is {self.header.steps} steps, but data size can contain only "
-                             f"{self.valid_steps} steps")
+            raise ValueError(f"frame count mismatch: the header declares {self.header.steps} frames, "
+                             f"the file holds {self.valid_steps} whole frames")
         if (self.file_size - HEADER_SIZE) % self.frame_bytes:
-            raise ValueError(f"Data size is {self.file_size}, but {self.valid_steps} step data have to be "
-                             f"{self.correct_file_size}")
+            extra = self.file_size - self.correct_file_size
+            raise ValueError(f"partial frame: {extra} bytes follow the last whole frame "
+                             f"({self.file_size} bytes in the file, {self.correct_file_size} expected)")
         return True
 
     def frames(self, start=0, stop=None):
@@ -118,10 +121,11 @@ def split(in_file, out_file, start, steps):
     with open(in_file, "rb") as inf, open(out_file, "wb") as outf:
         outf.write(h.pack())
         inf.seek(HEADER_SIZE + src.frame_bytes * (start - t0_in))
-        for _ in range(steps):
+        for k in range(steps):
             data = inf.read(src.frame_bytes)
             if len(data) != src.frame_bytes:
-                raise ValueError("data length is not suitable. check the data.")
+                raise ValueError(f"{in_file} ends before frame t={start + k} (holds t={t0_in}.."
+                                 f"{t0_in + src.valid_steps - 1})")
             outf.write(data)
 
 
@@ -178,11 +182,12 @@ def _main(argv):
         rc = 0
         for fn in a:
             p = PtclFile(fn)
+            summary = f"{fn}: N={p.N}, first frame t={p.header.t_0}, {p.header.steps} frames in the header"
             try:
                 p.check_metadata()
-                print(f"{fn}\n  N:{p.N} t0:{p.header.t_0} step:{p.header.steps}")
+                print(f"{summary}; ok")
             except ValueError as e:
-                print(f"{fn}\n  N:{p.N} t0:{p.header.t_0} step:{p.header.steps} Invalid metadata: {e}")
+                print(f"{summary}; BAD ({e})")
                 rc = 1
         return rc
     else:

@@ -145,6 +145,10 @@ def test_autotune_candidates_and_cache_key():
     # a cell scale that would leave < 3 cells per axis is dropped
     small = E.make_params(1000, 0.3, mode=E.MODE_GRID_F64)
     assert T.auto_cells(small) == 3
+    # the cells-per-boid cap is the exact integer cube root (perfect cubes included)
+    for n, g in ((64, 4), (4096, 16), (32768, 32), (1 << 18, 64), (4095, 15), (35937, 33), (35936, 32)):
+        assert T.icbrt(n) == g
+        assert T.auto_cells(E.make_params(n, 20.0, mode=E.MODE_GRID_F64)) == g
     assert all(t["cell_scale"] == 1.0 for t, _ in T.candidates(small))
     assert T.workload_key(p) == T.workload_key(E.make_params(1 << 20, 4.0, mode=E.MODE_GRID_MIXED))
     assert T.workload_key(p) != T.workload_key(E.make_params(1 << 20, 4.1, mode=E.MODE_GRID_MIXED))
