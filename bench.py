@@ -1,29 +1,40 @@
 #!/usr/bin/env python3
 """Boid-step throughput on one MI355X: FP64 plain vs mixed-precision grid path.
 
-Workload (default): N = 2^20 boids, cubic field F = 4.0 (density 16384 per
-unit volume), uniform positions, isotropic directions with |v| ~ U[0.001,
-0.005] (seed 2), rules: radii 0.02/0.05/0.08 (separation/alignment/
+Workload (default, flock1m): N = 2^20 boids, cubic field F = 4.0 (density
+16384 per unit volume), uniform positions, isotropic directions with |v| ~
+U[0.001, 0.005] (seed 2), rules: radii 0.02/0.05/0.08 (separation/alignment/
 cohesion), view angle 270 deg, weights 0.002/0.06/0.008, dt = 1, periodic
 wrap, T = 1000 updates.  A "step" is one full update() of all N boids: fused
 neighbour + integrate kernel, then the counting sort that re-bins the new
-positions.  The timed steps start from the initial state (the warmup steps
-run first and the state is then reset): cohesion clusters the flock as the
-run goes on (mean cohesion neighbours 35 at t = 0, ~118 at t = 1000), so a
-step's cost depends on where in the run it is, and the timed window is the
-configured run itself.
+positions.  Cohesion clusters the flock as the run goes on (mean cohesion
+neighbours 35 at t = 0, ~118 at t = 1000), so a step's cost depends on where
+in the run it is.
 
-Prints ONE JSON line.  `value` is the mixed-precision path's boid-steps/s
-with the state resident in HBM; the FP64 plain figure is reported beside it.
-`roofline` is for the dominant kernel (fused neighbour + integrate) of the
-headline path, from HIP events on the engine's stream (`traffic` and `valu`
-are read from the committed rocprofv3 summaries); `cpu_baseline` times
-the oracle's FP64 cell-list step (OpenMP) on this host for a bounded sample.
+Timed steps.  With --steps K >= T the timed region is the whole run from its
+initial state (K consecutive updates).  With K < T the run is stratified:
+the engine steps through the configured T updates once, and the K timed
+updates are the ones at t_k = floor((k + 1/2) T / K), k = 0..K-1 (the
+midpoints of K equal segments of the run); every timed update is bracketed
+by device synchronisations, the updates between them run untimed.  `value`
+= N K / (sum of the K timed update times): an estimate of the T-update run's
+throughput.  The first K updates from t = 0 are timed as well and reported
+as `from_t0` (the uniform start, the cheapest part of the run).
 
-`--workload flock16k` (N=16384, F=1, T=8000; the CPU beside it is the
-reference's own grid engine built from source, oracle/_ref/ref_grid) and
-`--workload blobs16m` (N=2^24, F=10, 64 clustered blobs, T=500, mixed)
-measure the same path at the other configured sizes.
+Prints ONE JSON line.  `value` is the headline path's boid-steps/s with the
+state resident in HBM (flock1m: mixed precision; the FP64 plain figure
+beside it).  `roofline` is for the dominant kernel (fused neighbour +
+integrate) of the headline path, from HIP events on the engine's stream
+over the timed updates (`traffic` and `valu` are read from the committed
+rocprofv3 summaries); `cpu_baseline` times the CPU path beside it on this
+host for a bounded sample, with all host threads and with one.
+
+Other workloads (--workload): flock16k (N=16384, F=1, T=8000, FP64 plain;
+the CPU beside it is the reference's own grid engine built from source,
+oracle/_ref/ref_grid), flock1k (N=1024, F=1, T=100, all-pairs FP64 and the
+reference-faithful float mode; CPU: the reference's own all-pairs engine,
+oracle/_ref/ref_allpairs) and blobs16m (N=2^24, F=10, 64 clustered blobs,
+T=500, mixed).
 
 `sweep` (with --sweep-steps > 0) reports the many-flock parameter sweep: 512
 flocks, N log-uniform in [2^10, 2^16], field (N/16384)^(1/3), vmax ~
@@ -51,15 +62,18 @@ import boidssimulatormpi_amd as B  # noqa: E402
 from boidssimulatormpi_amd import engine as E  # noqa: E402
 
 RULES = dict(sep=(0.02, 0.75, 0.002), ali=(0.05, 0.75, 0.06), coh=(0.08, 0.75, 0.008), vmax=0.005, vmin=0.001)
-# Workloads (all with RULES): the headline (default) and two more shapes of
-# the same path, each run from t = 0 for its configured number of updates.
+# Workloads (all with RULES): the headline (default) and three more shapes of
+# the same path; `steps` is the run's configured length T.
 WORKLOADS = {
     "flock1m": dict(n=1 << 20, field=4.0, seed=2, init=E.INIT_ISOTROPIC, steps=1000,
                     modes="grid_f64,grid_mixed", head="grid_mixed", cpu="port", sweep=True,
                     what="uniform positions, isotropic velocities"),
     "flock16k": dict(n=16384, field=1.0, seed=1, init=E.INIT_ISOTROPIC, steps=8000,
-                     modes="grid_f64,grid_mixed", head="grid_f64", cpu="reference", sweep=False,
+                     modes="grid_f64,grid_mixed", head="grid_f64", cpu="ref_grid", sweep=False,
                      what="uniform positions, isotropic velocities"),
+    "flock1k": dict(n=1024, field=1.0, seed=0, init=E.INIT_ISOTROPIC, steps=100,
+                    modes="allpairs_f64,allpairs_f32ref", head="allpairs_f64", cpu="ref_allpairs", sweep=False,
+                    what="uniform positions, isotropic velocities"),
     "blobs16m": dict(n=1 << 24, field=10.0, seed=4, init=E.INIT_CLUSTERED, steps=500,
                      modes="grid_mixed", head="grid_mixed", cpu=None, sweep=False,
                      what="64 Gaussian blobs (sigma 0.05 F), per-blob headings"),
@@ -97,7 +111,17 @@ def tuned_params(mode, mid, n, field, tune, block=0):
                    "worst_ms": max(t["ms_per_step"] for t in timings)}
 
 
-def run_mode(mode, init, n, field, steps, warmup, params):
+def sample_times(T, K):
+    """Timed updates of a stratified run: the midpoints of K equal segments
+    of the T-update run (all of it, in order, when K >= T)."""
+    if K >= T:
+        return list(range(K))
+    return [int((k + 0.5) * T / K) for k in range(K)]
+
+
+def run_mode(mode, init, n, field, T, steps, warmup, params):
+    """Times `steps` updates of one mode over the T-update run from `init`
+    (sample_times), then the first `steps` updates from t = 0."""
     e = B.Engine(params)
     info = e.info()
     def load():
@@ -107,29 +131,67 @@ def run_mode(mode, init, n, field, steps, warmup, params):
             e.set_state(xyzv6=init)
     load()
     e.step(warmup)
-    load()                           # the timed steps are the run from the start state
+    load()                           # the run starts from the initial state
     e.sync()
     e.reset_timing()
-    t0 = time.perf_counter()
-    e.step(steps)
-    e.sync()
-    t1 = time.perf_counter()
+    ts = sample_times(T, steps)
+    pair0 = (0.0, 0)
+    wall, pair_ms, pair_n, samples = 0.0, 0.0, 0, []
+    t_cur = 0
+    if steps >= T:                   # the whole run, consecutive updates
+        t0 = time.perf_counter()
+        e.step(steps)
+        e.sync()
+        wall = time.perf_counter() - t0
+        pair_ms, pair_n = e.timing(E.K_PAIR)
+        t_cur = steps
+    else:
+        for t in ts:
+            if t > t_cur:
+                e.step(t - t_cur)    # untimed
+            e.sync()
+            pair0 = e.timing(E.K_PAIR)
+            t0 = time.perf_counter()
+            e.step(1)
+            e.sync()
+            dt = time.perf_counter() - t0
+            p1 = e.timing(E.K_PAIR)
+            wall += dt
+            pair_ms += p1[0] - pair0[0]
+            pair_n += p1[1] - pair0[1]
+            samples.append([t, round(1e3 * dt, 4), round(p1[0] - pair0[0], 4)])
+            t_cur = t + 1
     kernels = {}
     for k, name in enumerate(E.K_NAMES):
         ms, cnt = e.timing(k)
         if cnt:
             kernels[name] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
-    # physical sanity of the end state (cheap, outside the timed region)
+    # physical sanity of the state reached (cheap, outside the timed region)
     p, v = e.state_f64()
-    s = np.linalg.norm(v, axis=1)
-    ok = bool(np.isfinite(p).all() and (s <= RULES["vmax"] * (1 + 1e-9)).all())
+    sp = np.linalg.norm(v, axis=1)
+    ok = bool(np.isfinite(p).all() and (sp <= RULES["vmax"] * (1 + 1e-9)).all())
     rechecks = e.rechecks()
+    # the first `steps` updates from the initial state (uniform start)
+    load()
+    e.sync()
+    e.reset_timing()
+    k0 = min(steps, T)
+    t0 = time.perf_counter()
+    e.step(k0)
+    e.sync()
+    w0 = time.perf_counter() - t0
+    ms0, c0 = e.timing(E.K_PAIR)
     e.close()
-    wall = t1 - t0
-    return {"mode": E.MODE_NAMES[mode], "cells_per_axis": info.cells_per_axis, "block": info.block,
-            "tile": info.tile, "fill": info.fill,
-            "ms_per_step": 1e3 * wall / steps, "boid_steps_per_s": n * steps / wall,
-            "kernels": kernels, "sane": ok, "fp64_rechecks": rechecks}
+    out = {"mode": E.MODE_NAMES[mode], "cells_per_axis": info.cells_per_axis, "block": info.block,
+           "tile": info.tile, "fill": info.fill, "timed_updates": len(ts),
+           "ms_per_step": 1e3 * wall / steps, "boid_steps_per_s": n * steps / wall,
+           "pair_us_avg": 1e3 * pair_ms / max(pair_n, 1), "kernels_all_updates": kernels, "sane": ok,
+           "fp64_rechecks": rechecks,
+           "from_t0": {"updates": f"t = 0..{k0 - 1}", "ms_per_step": 1e3 * w0 / k0, "boid_steps_per_s": n * k0 / w0,
+                       "pair_us_avg": 1e3 * ms0 / max(c0, 1)}}
+    if samples:
+        out["samples_t_ms_pairms"] = samples
+    return out
 
 
 def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
@@ -196,52 +258,95 @@ def run_sweep(steps, base_steps, warmup, count=512, seed=3, block=0):
     return out
 
 
-def cpu_model():
+def cpu_info():
+    """The host CPU as lscpu reports it, and the threads this process may use."""
+    info = {"threads_available": len(os.sched_getaffinity(0))}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
-        return next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+        keys = {"Model name": "model", "Vendor ID": "vendor", "CPU family": "family", "Model": "model_id",
+                "Stepping": "stepping", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "CPU max MHz": "max_mhz",
+                "L3": "l3", "Hypervisor vendor": "hypervisor"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in keys and keys[k] not in info:
+                info[keys[k]] = v.strip()
     except Exception:
-        return ""
+        pass
+    return info
 
 
-def cpu_reference_grid(n, field, seed, budget_s):
-    """The reference's own grid engine (src/boid_simulation_grid.cpp, compiled
-    from source into oracle/_ref/ref_grid by oracle/Makefile.ref) on the same
-    shape of workload (its random_uniform init): per-update time from two runs
-    of different length, so setup and init cancel.  None when not built."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_grid")
-    if not os.path.exists(exe):
-        return None
-    threads = len(os.sched_getaffinity(0))
+def _ref_update_seconds(exe, n, field, init, seed, threads, budget_s):
+    """Seconds per update() of a reference engine (oracle/_ref/*, built from the
+    reference's own sources by oracle/Makefile.ref): two runs of different
+    length, so setup and init cancel.  Returns (seconds, updates timed)."""
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = RULES
     def run(t):
         args = [exe, os.devnull, str(n), str(t), repr(field), *(repr(x) for x in (*r["sep"], *r["ali"], *r["coh"])),
-                repr(r["vmax"]), repr(r["vmin"]), "random_uniform", str(seed), ""]
+                repr(r["vmax"]), repr(r["vmin"]), init, str(seed), ""]
         t0 = time.perf_counter()
-        subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL, timeout=600)
+        subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL, timeout=900)
         return time.perf_counter() - t0
     w1 = run(1)
-    per = max(run(2) - w1, 1e-3)                         # one update, for sizing
+    per = max(run(2) - w1, 1e-4)                         # one update, for sizing
     k = max(2, int(budget_s / per))
     wk = run(1 + k)
-    per = (wk - w1) / k
+    return (wk - w1) / k, k
+
+
+def cpu_reference(which, n, field, seed, init, budget_s):
+    """The reference's own engine on the same workload: ref_grid (boidsim_grid,
+    src/boid_simulation_grid.cpp, with its own random_uniform init at `seed`)
+    or ref_allpairs (boidsim, src/boid_simulation.cpp, reading `init` from a
+    file), all host threads and one.  None when not built."""
+    exe = os.path.join(ROOT, "oracle", "_ref", which)
+    if not os.path.exists(exe):
+        return None
+    tmp = None
+    if which == "ref_allpairs":
+        import tempfile
+        fd, tmp = tempfile.mkstemp(suffix=".init")
+        os.close(fd)
+        np.ascontiguousarray(init, "<f4").tofile(tmp)
+        how = tmp
+    else:
+        how = "random_uniform"
+    threads = len(os.sched_getaffinity(0))
+    try:
+        per, k = _ref_update_seconds(exe, n, field, how, seed, threads, budget_s)
+        per1, k1 = _ref_update_seconds(exe, n, field, how, seed, 1, budget_s / 2)
+    finally:
+        if tmp:
+            os.unlink(tmp)
+    src = "src/boid_simulation_grid.cpp" if which == "ref_grid" else "src/boid_simulation.cpp"
     return {"value": n / per, "unit": "boid-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"{k} updates of the reference grid engine (oracle/_ref/ref_grid, src/boid_simulation_grid.cpp "
-                      f"built from source), N={n}, F={field}, random_uniform init seed {seed}, {threads} OpenMP "
-                      f"threads: (time of 1+{k} updates - time of 1) / {k}; CPU: {cpu_model()}"}
+            "sample": f"{k} updates of oracle/_ref/{which} ({src} built from source), N={n}, {threads} threads",
+            "value_1thread": n / per1, "sample_1thread": f"{k1} updates, 1 thread",
+            "method": "(time of 1+k updates - time of 1 update) / k; init " +
+                      ("the reference's random_uniform at the workload seed" if which == "ref_grid"
+                       else "the workload's own initial state as an init file"),
+            "cpu": cpu_info()}
 
 
-def cpu_baseline(mid, t_mid, n, field, budget_s):
-    """Oracle FP64 cell-list step (the reference algorithm restated, OpenMP)
-    on the same workload, from its state at t = t_mid (a step's cost grows as
-    the flock clusters, so the sample starts mid-run); steps until ~budget_s
-    of wall time."""
+def cpu_port(mid, t_mid, n, field, budget_s):
+    """The oracle's FP64 cell-list update (the reference algorithm restated,
+    OpenMP) on the same workload from its state at t = t_mid: full updates on
+    all host threads until ~budget_s, and one thread on a sample of owners
+    (phase A of every stride-th boid, scaled to all N, plus the timed cell-list
+    build and phase B)."""
     from oracle import oracle as O
     P = O.params(field, RULES["sep"], RULES["ali"], RULES["coh"], RULES["vmax"], RULES["vmin"])
     p = np.ascontiguousarray(mid[0], np.float64).copy()
     v = np.ascontiguousarray(mid[1], np.float64).copy()
     threads = len(os.sched_getaffinity(0))
+    # one thread first (the full updates below change the state)
+    k, tb, ta, tp = O.time_step_sample(P, p, v, 1024, 1)
+    want = max(1.0, budget_s / 2)
+    stride = max(1, int(1024 * ta / want))
+    k, tb, ta, tp = O.time_step_sample(P, p, v, stride, 1)
+    per1 = tb + tp + ta * n / k
     steps, t0 = 0, time.perf_counter()
     while True:
         O.step(P, p, v, enumeration=O.ENUM_CELLS, nthreads=threads)
@@ -249,10 +354,13 @@ def cpu_baseline(mid, t_mid, n, field, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     wall = time.perf_counter() - t0
-    model = cpu_model()
     return {"value": n * steps / wall, "unit": "boid-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} full step(s) of the N={n} workload from its t={t_mid} state, oracle FP64 cell list, "
-                      f"{threads} OpenMP threads, {wall:.1f} s wall; CPU: {model}"}
+            "sample": f"{steps} full updates from the t={t_mid} state, oracle FP64 cell list, {threads} threads, "
+                      f"{wall:.1f} s",
+            "value_1thread": n / per1,
+            "sample_1thread": f"1 thread: phase A of every {stride}th owner ({k} owners, {ta:.1f} s) x {n / k:.1f}, "
+                              f"+ cell-list build {tb:.2f} s + phase B {tp:.2f} s",
+            "cpu": cpu_info()}
 
 
 def traffic_from_profiles(mode_name, n):
@@ -295,8 +403,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="flock1m", choices=sorted(WORKLOADS),
                     help="flock1m: the headline (N=2^20, F=4); flock16k: N=16384, F=1, FP64 plain vs the "
-                         "reference's own grid engine on the CPU; blobs16m: N=2^24, F=10, 64 clustered blobs")
-    ap.add_argument("--steps", type=int, default=None, help="timed updates (default: the workload's)")
+                         "reference's own grid engine on the CPU; flock1k: N=1024 all-pairs vs the reference's "
+                         "own all-pairs engine; blobs16m: N=2^24, F=10, 64 clustered blobs")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed updates (default: the whole run); fewer than the run's length are stratified "
+                         "over the run")
+    ap.add_argument("--run-length", type=int, default=0,
+                    help="updates in the run (default: the workload's configured T)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--from-t", type=int, default=0,
                     help="start the timed updates from the workload's state after this many updates (FP64 "
@@ -326,13 +439,15 @@ def main():
     n = a.n if a.n is not None else wl["n"]
     F = a.field if a.field is not None else wl["field"]
     seed = a.seed if a.seed is not None else wl["seed"]
-    a.steps = a.steps if a.steps is not None else wl["steps"]
+    T = a.run_length if a.run_length else wl["steps"]
+    a.steps = a.steps if a.steps is not None else T
     a.modes = a.modes or wl["modes"]
     if a.sweep_steps is None:
         a.sweep_steps = 100 if wl["sweep"] else 0
     init = B.generate_init(wl["init"], n, F, RULES["vmax"], RULES["vmin"], seed=seed)
     by_name = {v: k for k, v in E.MODE_NAMES.items()}
-    t_mid = min(500, a.steps // 2) if wl["init"] == E.INIT_ISOTROPIC else 0
+    T_run = max(1, T - a.from_t)                     # updates from the start state to the run's end
+    t_mid = T // 2
     mid = None
     if a.tune or (not a.no_cpu and wl["cpu"] == "port"):
         log(f"[bench] mid-run state (t={t_mid}) for tuning and the CPU sample")
@@ -342,19 +457,22 @@ def main():
     if a.from_t > 0:
         log(f"[bench] start state: t={a.from_t}")
         start = mid_run_state(init, n, F, a.from_t)
+    ts = sample_times(T_run, a.steps)
     for name in a.modes.split(","):
-        log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} steps={a.steps}")
+        log(f"[bench] {name}: N={n} F={F} warmup={a.warmup} timed updates={a.steps} of a {T_run}-update run")
         params, tuning = tuned_params(by_name[name], mid, n, F, a.tune, a.block)
-        res[name] = run_mode(by_name[name], start, n, F, a.steps, a.warmup, params)
+        res[name] = run_mode(by_name[name], start, n, F, T_run, a.steps, a.warmup, params)
         res[name]["tuning"] = tuning
-        log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} boid-steps/s")
+        log(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['boid_steps_per_s']:.3e} "
+            f"boid-steps/s (from t=0 only: {res[name]['from_t0']['ms_per_step']:.3f} ms/step)")
 
     head = res.get(wl["head"]) or next(iter(res.values()))
-    pair = head["kernels"].get("pair", {"us_avg": float("nan")})
-    achieved = BYTES_PER_BOID_STEP * n / (pair["us_avg"] * 1e-6) / 1e9
+    bytes_per = BYTES_PER_BOID_STEP // 2 if head["mode"] == "allpairs_f32ref" else BYTES_PER_BOID_STEP
+    achieved = bytes_per * n / (head["pair_us_avg"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(head["mode"], n),
-                "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": BYTES_PER_BOID_STEP,
+                "kernel": f"fused neighbour+integrate ({head['mode']})", "bytes_per_boid_step": bytes_per,
+                "kernel_us_avg": head["pair_us_avg"], "over": "the timed updates (HIP events on the engine's stream)",
                 "valu": valu_from_profiles(head["mode"], n)}
     sweep = None
     if a.sweep_steps > 0:
@@ -362,11 +480,18 @@ def main():
         sweep = run_sweep(a.sweep_steps, a.sweep_base_steps, 5, block=a.block)
     cpu = None
     if not a.no_cpu and wl["cpu"]:        # blobs16m: one oracle step alone exceeds the sample budget
-        log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s)")
-        cpu = (cpu_reference_grid(n, F, seed, a.cpu_seconds) if wl["cpu"] == "reference"
-               else cpu_baseline(mid, t_mid, n, F, a.cpu_seconds))
+        log(f"[bench] cpu baseline (~{a.cpu_seconds:.0f} s all threads + ~{a.cpu_seconds / 2:.0f} s one thread)")
+        cpu = (cpu_port(mid, t_mid, n, F, a.cpu_seconds) if wl["cpu"] == "port"
+               else cpu_reference(wl["cpu"], n, F, seed, init, a.cpu_seconds))
     metric = ("boid-steps/sec at N=2^20: FP64 plain vs mixed precision; % HBM roofline" if a.workload == "flock1m"
               else f"boid-steps/sec, {a.workload} (N={n}, {head['mode']})")
+    if a.steps >= T_run:
+        timed = f"the whole {a.steps}-update run from t = {a.from_t}, consecutive updates"
+    else:
+        timed = (f"{a.steps} single updates stratified over the {T_run}-update run from t = {a.from_t} "
+                 f"(t = {ts[0] + a.from_t}, {ts[1] + a.from_t}, ..., {ts[-1] + a.from_t}: midpoints of "
+                 f"{a.steps} equal segments), each bracketed by device syncs, the rest untimed")
+    dtype = {"grid_mixed": "f32+f64", "allpairs_f32ref": "f32"}.get(head["mode"], "f64")
     out = {
         "metric": metric,
         "value": head["boid_steps_per_s"],
@@ -378,15 +503,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head["mode"] == "grid_mixed" else "f64",
+        "dtype": dtype,
         "data": f"synthetic ({wl['what']}, seed {seed})",
-        "config": {"workload": f"grid boid step, N={n}, F={F}, {wl['what']}, |v| in [0.001,0.005], radii "
-                               f"0.02/0.05/0.08, view 270 deg, weights 0.002/0.06/0.008, seed {seed}, timed updates "
-                               f"t = {a.from_t}..{a.from_t + a.steps - 1}",
-                   "name": a.workload, "n": n, "field": F, "timed_from_t": a.from_t, "headline_mode": head["mode"],
-                   "parallelism": "single GPU"},
-        "fp64_plain": res.get("grid_f64"),
-        "mixed": res.get("grid_mixed"),
+        "config": {"workload": f"boid update, N={n}, F={F}, {wl['what']}, |v| in [0.001,0.005], radii "
+                               f"0.02/0.05/0.08, view 270 deg, weights 0.002/0.06/0.008, seed {seed}, "
+                               f"a {T}-update run",
+                   "timed": timed, "name": a.workload, "n": n, "field": F, "run_length": T,
+                   "timed_from_t": a.from_t, "headline_mode": head["mode"], "parallelism": "single GPU"},
+        "value_from_t0": head["from_t0"]["boid_steps_per_s"],
+        "modes": res,
         "mixed_over_fp64": (res["grid_mixed"]["boid_steps_per_s"] / res["grid_f64"]["boid_steps_per_s"]
                             if "grid_mixed" in res and "grid_f64" in res else None),
         "roofline": roofline,

@@ -7,7 +7,7 @@
 #include <cstring>
 
 #ifdef BS_VARIANT_GRID
-static const int kDefaultMode = BS_MODE_GRID_F64;
+static const int kDefaultMode = BS_MODE_GRID_F32REF;   // the reference's float arithmetic, bit for bit
 #else
 static const int kDefaultMode = BS_MODE_ALLPAIRS_F32REF;
 #endif
@@ -20,6 +20,7 @@ static int mode_from_env(int fallback)
     if (!std::strcmp(m, "allpairs_f64")) return BS_MODE_ALLPAIRS_F64;
     if (!std::strcmp(m, "grid_f64")) return BS_MODE_GRID_F64;
     if (!std::strcmp(m, "grid_mixed")) return BS_MODE_GRID_MIXED;
+    if (!std::strcmp(m, "grid_f32ref")) return BS_MODE_GRID_F32REF;
     std::cerr << "unknown BOIDS_ENGINE_MODE: " << m << std::endl;
     std::exit(-1);
 }
@@ -72,14 +73,21 @@ void BoidSimulation::setup(unsigned int number_of_agents, double field_size, dou
     if (bs_create(&p, &engine_) != BS_OK) fatal("engine setup failed");
     bs_info info;
     bs_get_info(engine_, &info);
-    mode_ = info.mode;
     GRID_NUM = (unsigned)info.cells_per_axis;
     pos_.assign(3 * (size_t)N, 0.f);
     vel_.assign(3 * (size_t)N, 0.f);
     frc_.assign(9 * (size_t)N, 0.f);
-    if (mode_ == BS_MODE_GRID_F64 || mode_ == BS_MODE_GRID_MIXED)
+    if (grid_mode())
         std::cout << "# Grid List Optimization enabled" << std::endl
                   << "  grid num:" << GRID_NUM << std::endl;
+}
+
+// The grid variants behave as the reference's grid engine (srand48(seed) at
+// init, src/boid_simulation_grid.cpp:98), whichever enumeration the engine
+// falls back to on a small field.
+bool BoidSimulation::grid_mode() const
+{
+    return mode_ == BS_MODE_GRID_F64 || mode_ == BS_MODE_GRID_MIXED || mode_ == BS_MODE_GRID_F32REF;
 }
 
 // src/boid_simulation.cpp:79-137 (all-pairs: drand48 never seeded) and
@@ -87,7 +95,7 @@ void BoidSimulation::setup(unsigned int number_of_agents, double field_size, dou
 void BoidSimulation::init()
 {
     std::vector<float> st(6 * (size_t)N);
-    const bool grid = mode_ == BS_MODE_GRID_F64 || mode_ == BS_MODE_GRID_MIXED;
+    const bool grid = grid_mode();
     int rc;
     if (initialization_type == "test" || initialization_type == "random_uniform") {
         const int kind = initialization_type == "test" ? BS_INIT_TEST : BS_INIT_RANDOM_UNIFORM;

@@ -62,7 +62,9 @@ public:
 
     unsigned int GRID_NUM;     // cells per axis of the grid variant (0: all-pairs)
 
-    // Engine mode actually running (BS_MODE_*), for diagnostics.
+    // Engine mode asked for (BS_MODE_*, BOIDS_ENGINE_MODE or the variant's
+    // default), for diagnostics; a grid mode may run all-pairs on a field
+    // with fewer than 3 cells per axis (GRID_NUM 0 then).
     int engine_mode() const { return mode_; }
     // Ask the engine to keep per-boid forces for get_force (call before setup()).
     void enable_force_output(bool on) { want_forces_ = on; }
@@ -74,6 +76,7 @@ protected:
 private:
     void fatal(const std::string& what) const;
     void refresh() const;
+    bool grid_mode() const;
 
     bs_engine* engine_ = nullptr;
     int mode_;

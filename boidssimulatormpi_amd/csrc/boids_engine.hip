@@ -97,7 +97,9 @@ struct bs_engine {
     bool capturing = false;
     Lattice lat{};
     double2* rec = nullptr;            // FP64 grid: interleaved pair records
+    float4 *rec4 = nullptr, *vrec4 = nullptr;   // float grid: sorted positions (+ id) and velocities
     uint32_t* recheck = nullptr;
+    uint32_t* overflow = nullptr;      // chunk-table count beyond its capacity (0: none)
     // outputs
     void* forces = nullptr;   // float or double, 9n, id order
     uint64_t* digest = nullptr;
@@ -535,6 +537,8 @@ static int alloc_f32(bs_engine* e, StateF32* s, uint32_t n)
 }
 
 static bool is_grid(int mode) { return mode == BS_MODE_GRID_F64 || mode == BS_MODE_GRID_MIXED; }
+// float state in id order (the reference-faithful modes)
+static bool is_f32(int mode) { return mode == BS_MODE_ALLPAIRS_F32REF || mode == BS_MODE_GRID_F32REF; }
 
 static Flocks flock_view(const bs_engine* e)
 {
@@ -555,7 +559,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
 {
     const bs_params* p = &fp[0];
     if (!(p->field > 0.0) || !isfinite(p->field)) return fail(nullptr, BS_E_ARG, "field size must be positive and finite");
-    if (p->mode < BS_MODE_ALLPAIRS_F32REF || p->mode > BS_MODE_GRID_MIXED) return fail(nullptr, BS_E_ARG, "unknown mode");
+    if (p->mode < BS_MODE_ALLPAIRS_F32REF || p->mode > BS_MODE_GRID_F32REF) return fail(nullptr, BS_E_ARG, "unknown mode");
     if (batch && !is_grid(p->mode)) return fail(nullptr, BS_E_ARG, "bs_create_batch: grid modes only");
     uint64_t ntot = 0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -660,6 +664,22 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             e->flocks.clear();
             e->flock_field.clear();
         }
+    } else if (e->mode == BS_MODE_GRID_F32REF) {
+        // plain cells (edge >= the largest radius), >= 3 per axis so the 27
+        // neighbour cells are distinct; else all-pairs, which gives the same
+        // bits (the reference's grid engine equals its all-pairs engine)
+        const int G = cells_per_axis(p);
+        if (G < 0) { release(e); return fail(nullptr, BS_E_ARG, "cells_per_axis must be >= 3 with edge >= max radius"); }
+        if (G < 3) {
+            e->mode = BS_MODE_ALLPAIRS_F32REF;
+        } else {
+            e->G = G;
+            e->edge = p->field / G;
+            e->P.G = G;
+            e->P.sx = 1;
+            e->P.inv_edge = G / p->field;
+            e->ncell1 = (uint32_t)((uint64_t)G * G * G + 1);
+        }
     }
 
     hipError_t st = hipSetDevice(p->device);
@@ -670,7 +690,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
 
     const uint32_t n = e->prm.n;
     auto bail = [&](int code) { std::string m = e->err; release(e); g_create_error = m; return code; };
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         if ((rc = alloc_f32(e, &e->fa, n)) || (rc = alloc_f32(e, &e->fb, n))) return bail(rc);
         if (p->flags & BS_FLAG_FORCES) { float* f; if ((rc = dev_alloc(e, &f, 9 * (size_t)n))) return bail(rc); e->forces = f; }
     } else {
@@ -680,6 +700,16 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     }
     if ((rc = dev_alloc(e, &e->tmp_pos, 3 * (size_t)n)) || (rc = dev_alloc(e, &e->tmp_vel, 3 * (size_t)n))) return bail(rc);
     if (p->flags & BS_FLAG_DIGEST) { if ((rc = dev_alloc(e, &e->digest, 4 * (size_t)n))) return bail(rc); }
+    if (e->mode == BS_MODE_GRID_F32REF) {
+        const uint32_t ntiles = (e->ncell1 + 4095) / 4096;
+        if ((rc = dev_alloc(e, &e->key, n)) || (rc = dev_alloc(e, &e->rank, n)) || (rc = dev_alloc(e, &e->perm, n)) ||
+            (rc = dev_alloc(e, &e->count, e->ncell1)) || (rc = dev_alloc(e, &e->start, e->ncell1)) ||
+            (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->rec4, n)) ||
+            (rc = dev_alloc(e, &e->vrec4, n)))
+            return bail(rc);
+        st = hipMemsetAsync(e->count, 0, sizeof(uint32_t) * e->ncell1, e->stream);
+        if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
+    }
     if (is_grid(e->mode)) {
         // full chunks <= n / chunk; groups counts each (row, span group)'s last chunk and short cuts (k_rows_pass)
         if ((uint64_t)(n + e->chunk - 1) / e->chunk + e->groups + 1 > 0x7fffffffull) return bail(fail(e, BS_E_ARG, "chunk table too large"));
@@ -690,6 +720,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             (rc = dev_alloc(e, &e->partials, ntiles)) || (rc = dev_alloc(e, &e->nchunks, 1)) ||
             (rc = dev_alloc(e, &e->perm, n)) || (rc = dev_alloc(e, &e->permid, n)) || (rc = dev_alloc(e, &e->rank, n)) ||
             (rc = dev_alloc(e, &e->table, e->max_chunks)) || (rc = dev_alloc(e, &e->recheck, 1)) ||
+            (rc = dev_alloc(e, &e->overflow, 1)) ||
             (rc = dev_alloc(e, &e->row_count, (size_t)e->rows + 1)) ||
             (rc = dev_alloc(e, &e->row_off, (size_t)e->rows + 1)) ||
             (rc = dev_alloc(e, &e->flocks_dev, e->flocks.size())))
@@ -718,10 +749,11 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
         }
         st = hipMemsetAsync(e->count, 0, sizeof(uint32_t) * e->ncell1, e->stream);
         if (st == hipSuccess) st = hipMemsetAsync(e->recheck, 0, sizeof(uint32_t), e->stream);
+        if (st == hipSuccess) st = hipMemsetAsync(e->overflow, 0, sizeof(uint32_t), e->stream);
         if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
     }
     if (e->forces) {
-        size_t bytes = 9 * (size_t)n * (e->mode == BS_MODE_ALLPAIRS_F32REF ? sizeof(float) : sizeof(double));
+        size_t bytes = 9 * (size_t)n * (is_f32(e->mode) ? sizeof(float) : sizeof(double));
         st = hipMemsetAsync(e->forces, 0, bytes, e->stream);
         if (st != hipSuccess) { hip_fail(e, st, "hipMemsetAsync"); return bail(BS_E_HIP); }
     }
@@ -797,6 +829,7 @@ static int grid_sort_into_a(bs_engine* e)
     Timed t(e, BS_K_SCAN);
     HIP_TRY(e, launch_scan(e->count, e->start, e->ncell1, e->partials, e->stream));
     HIP_TRY(e, launch_rows(flock_view(e), e->start, e->row_count, e->row_off, e->partials, e->table, e->nchunks,
+                           e->max_chunks, e->overflow,
                            (int)e->chunk, e->cspan, e->sx * e->max_g, e->stream));
     t.stop();
     Timed t2(e, BS_K_SCATTER);
@@ -846,7 +879,7 @@ extern "C" int bs_set_state_f64(bs_engine* e, const double* pos3, const double* 
 {
     if (!e) return BS_E_ARG;
     if (e->prm.n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_set_state_f64: NULL array");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) return fail(e, BS_E_STATE, "float-state mode: use bs_set_state_f32");
+    if (is_f32(e->mode)) return fail(e, BS_E_STATE, "float-state mode: use bs_set_state_f32");
     return upload_f64(e, pos3, vel3);
 }
 
@@ -855,7 +888,7 @@ extern "C" int bs_set_state_f32(bs_engine* e, const float* xyzv6)
     if (!e) return BS_E_ARG;
     const uint32_t n = e->prm.n;
     if (n && !xyzv6) return fail(e, BS_E_ARG, "bs_set_state_f32: NULL array");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         std::vector<float> buf(6 * (size_t)n);
         for (uint32_t i = 0; i < n; ++i)
             for (int k = 0; k < 6; ++k) buf[(size_t)k * n + i] = xyzv6[6 * (size_t)i + k];
@@ -876,13 +909,29 @@ extern "C" int bs_set_state_f32(bs_engine* e, const float* xyzv6)
     return upload_f64(e, pos.data(), vel.data());
 }
 
+// Reports a chunk table cut short by its capacity (k_rows_pass clamps the
+// count and leaves the full one in e->overflow): the update then skipped
+// boids, so the state is not a valid result.  The capacity bound makes this
+// unreachable; the check runs wherever the host reads results back.
+static int check_grid(bs_engine* e)
+{
+    if (!e->overflow) return BS_OK;
+    uint32_t v = 0;
+    HIP_TRY(e, hipMemcpyAsync(&v, e->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    if (v)
+        return fail(e, BS_E_STATE, "chunk table overflow: " + std::to_string(v) + " chunks for a capacity of " +
+                                       std::to_string(e->max_chunks) + " (results invalid)");
+    return BS_OK;
+}
+
 extern "C" int bs_get_state_f64(bs_engine* e, double* pos3, double* vel3)
 {
     if (!e) return BS_E_ARG;
     const uint32_t n = e->prm.n;
     if (n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_get_state_f64: NULL array");
     if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         std::vector<float> p(3 * (size_t)n), v(3 * (size_t)n);
         int rc = bs_get_state_f32(e, p.data(), v.data());
         if (rc) return rc;
@@ -893,7 +942,7 @@ extern "C" int bs_get_state_f64(bs_engine* e, double* pos3, double* vel3)
     HIP_TRY(e, hipMemcpyAsync(pos3, e->tmp_pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(e, hipMemcpyAsync(vel3, e->tmp_vel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
-    return BS_OK;
+    return check_grid(e);
 }
 
 extern "C" int bs_get_state_f32(bs_engine* e, float* pos3, float* vel3)
@@ -902,7 +951,7 @@ extern "C" int bs_get_state_f32(bs_engine* e, float* pos3, float* vel3)
     const uint32_t n = e->prm.n;
     if (n && (!pos3 || !vel3)) return fail(e, BS_E_ARG, "bs_get_state_f32: NULL array");
     if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         std::vector<float> buf(6 * (size_t)n);
         float* src[6] = {e->fa.x, e->fa.y, e->fa.z, e->fa.vx, e->fa.vy, e->fa.vz};
         for (int k = 0; k < 6; ++k)
@@ -928,7 +977,7 @@ extern "C" int bs_get_forces_f64(bs_engine* e, double* f9)
     const uint32_t n = e->prm.n;
     if (!e->forces) return fail(e, BS_E_STATE, "engine created without BS_FLAG_FORCES");
     if (n && !f9) return fail(e, BS_E_ARG, "bs_get_forces_f64: NULL array");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         std::vector<float> tmp(9 * (size_t)n);
         HIP_TRY(e, hipMemcpyAsync(tmp.data(), e->forces, sizeof(float) * 9 * n, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(e, hipStreamSynchronize(e->stream));
@@ -946,7 +995,7 @@ extern "C" int bs_get_forces_f32(bs_engine* e, float* f9)
     const uint32_t n = e->prm.n;
     if (!e->forces) return fail(e, BS_E_STATE, "engine created without BS_FLAG_FORCES");
     if (n && !f9) return fail(e, BS_E_ARG, "bs_get_forces_f32: NULL array");
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF) {
+    if (is_f32(e->mode)) {
         HIP_TRY(e, hipMemcpyAsync(f9, e->forces, sizeof(float) * 9 * n, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(e, hipStreamSynchronize(e->stream));
         return BS_OK;
@@ -964,6 +1013,38 @@ extern "C" int bs_get_digest(bs_engine* e, uint64_t* d4)
     if (!e->digest) return fail(e, BS_E_STATE, "engine created without BS_FLAG_DIGEST");
     if (e->prm.n && !d4) return fail(e, BS_E_ARG, "bs_get_digest: NULL array");
     HIP_TRY(e, hipMemcpyAsync(d4, e->digest, 32 * (size_t)e->prm.n, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return check_grid(e);
+}
+
+// Grid built by the last sort (grid modes): sorted cell keys and boid ids,
+// cell starts and the chunk table, for exactness tests of the build.
+extern "C" int bs_debug_grid(bs_engine* e, uint32_t* key_sorted, uint32_t* id_sorted, uint32_t* start,
+                             uint32_t* table4, bs_grid_info* info)
+{
+    if (!e || !info) return BS_E_ARG;
+    memset(info, 0, sizeof(*info));
+    if (!is_grid(e->mode)) return fail(e, BS_E_STATE, "bs_debug_grid: not a grid engine");
+    if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
+    const uint32_t n = e->prm.n;
+    info->n = n;
+    info->ncells = e->ncell1 - 1;
+    info->max_chunks = e->max_chunks;
+    info->x_columns_per_cell = e->sx;
+    info->chunk_boids = (int32_t)e->chunk;
+    info->chunk_span = e->cspan;
+    HIP_TRY(e, hipMemcpyAsync(&info->nchunks, e->nchunks, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(&info->overflow, e->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    if (key_sorted)
+        HIP_TRY(e, hipMemcpyAsync(key_sorted, e->key_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream));
+    if (id_sorted)
+        HIP_TRY(e, hipMemcpyAsync(id_sorted, e->da.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e->stream));
+    if (start)
+        HIP_TRY(e, hipMemcpyAsync(start, e->start, sizeof(uint32_t) * e->ncell1, hipMemcpyDeviceToHost, e->stream));
+    if (table4 && info->nchunks)
+        HIP_TRY(e, hipMemcpyAsync(table4, e->table, sizeof(uint4) * std::min(info->nchunks, e->max_chunks),
+                                  hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     return BS_OK;
 }
@@ -1033,6 +1114,18 @@ extern "C" int bs_step(bs_engine* e, uint32_t nsteps)
             std::swap(e->fa, e->fb);
             break;
         }
+        case BS_MODE_GRID_F32REF: {
+            Timed ts(e, BS_K_SCAN);
+            HIP_TRY(e, launch_f32_grid_sort(e->P, e->fa, e->key, e->rank, e->count, e->start, e->partials, e->perm,
+                                            e->rec4, e->vrec4, e->stream));
+            ts.stop();
+            Timed t(e, BS_K_PAIR);
+            HIP_TRY(e, launch_f32_grid_step(e->P, e->fa, e->fb, e->rec4, e->vrec4, e->key, e->start,
+                                            (float*)e->forces, e->digest, e->stream));
+            t.stop();
+            std::swap(e->fa, e->fb);
+            break;
+        }
         case BS_MODE_ALLPAIRS_F64: {
             Timed t(e, BS_K_PAIR);
             HIP_TRY(e, launch_allpairs_f64(e->P, e->da, e->db, (double*)e->forces, e->digest, e->stream));
@@ -1058,6 +1151,11 @@ extern "C" int bs_sync(bs_engine* e)
 {
     if (!e) return BS_E_ARG;
     HIP_TRY(e, hipStreamSynchronize(e->stream));
+#ifdef BS_DEBUG_CHECKS
+    // debug builds check the chunk table after every synchronisation
+    int rc = check_grid(e);
+    if (rc) return rc;
+#endif
     return resolve_timing(e);
 }
 
@@ -1102,7 +1200,7 @@ extern "C" int bs_debug_thresholds(const bs_params* p, double* out6)
     std::string why;
     int rc = derive_thresholds(p, &P, &why);
     if (rc) return fail(nullptr, rc, why);
-    const bool f32 = p->mode == BS_MODE_ALLPAIRS_F32REF;
+    const bool f32 = is_f32(p->mode);
     for (int k = 0; k < 3; ++k) {
         out6[k] = f32 ? (double)P.s_thr_f[k] : P.s_thr[k];
         out6[3 + k] = P.cone[k] ? (f32 ? (double)P.c_thr_f[k] : P.c_thr[k]) : -2.0;
@@ -1175,7 +1273,7 @@ extern "C" int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32
     const double cap = std::max(1.0, 2.0 * floor(cbrt((double)n)));
     g = std::min(std::min(g, cap), 1024.0);
     const int G = std::max(1, (int)g);
-    if (e->mode == BS_MODE_ALLPAIRS_F32REF)
+    if (is_f32(e->mode))
         HIP_TRY(e, launch_f32_to_f64(ntot, e->fa, e->tmp_pos, e->tmp_vel, e->stream));
     else
         HIP_TRY(e, launch_unsort_f64(ntot, e->da, e->tmp_pos, e->tmp_vel, e->stream));

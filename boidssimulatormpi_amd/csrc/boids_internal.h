@@ -115,10 +115,11 @@ hipError_t launch_hash_count(Flocks F, StateF64 st, uint32_t* key, uint32_t* ran
 hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t n, uint32_t* partials,
                        hipStream_t stream);
 // chunk = most boids per chunk, cspan = most x columns a chunk covers,
-// max_cols = x columns of the longest row (sx * G over the flocks).
+// max_cols = x columns of the longest row (sx * G over the flocks), cap =
+// entries `table` holds (a count beyond it is clamped and left in *overflow).
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan, int max_cols,
-                       hipStream_t stream);
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, uint32_t cap, uint32_t* overflow,
+                       int chunk, int cspan, int max_cols, hipStream_t stream);
 hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
                           const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
 // Scatter by the arrival ranks the counting pass recorded (no atomics).
@@ -143,6 +144,15 @@ hipError_t launch_nbr_mixed(Flocks F, int nb, int tile, int max_cols, StateF64 i
                             uint32_t* count, double* forces, uint64_t* digest, uint32_t* recheck,
                             hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
+// Reference-faithful float grid (boids_f32grid.hip): the cell list of the
+// float state (keys, counting sort by (cell, id), sorted records), then the
+// step from it (id order in, id order out).
+hipError_t launch_f32_grid_sort(const DevParams& P, StateF32 st, uint32_t* key, uint32_t* rank, uint32_t* count,
+                                uint32_t* start, uint32_t* partials, uint32_t* perm, float4* rec, float4* vrec,
+                                hipStream_t stream);
+hipError_t launch_f32_grid_step(const DevParams& P, StateF32 in, StateF32 out, const float4* rec,
+                                const float4* vrec, const uint32_t* key, const uint32_t* start, float* forces,
+                                uint64_t* digest, hipStream_t stream);
 // flock metrics (boids_metrics.hip)
 size_t metrics_workspace_bytes(uint32_t n, int G);
 // Workspace regions are carved from `ws` (metrics_workspace_bytes long);

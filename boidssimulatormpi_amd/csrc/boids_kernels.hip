@@ -184,18 +184,29 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* start, uint
 // adjacent in the table).
 constexpr int kRowWaves = 4;
 
+//
+// The bound is the table's capacity (`cap`); pass 2 writes no entry at or
+// beyond it, and should the count ever exceed it, *nchunks is clamped to
+// the capacity and the full count is left in *overflow (non-zero marks the
+// table as cut short; the host reports it as an error).
 template <bool WRITE>
 __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const uint32_t* __restrict__ start,
                                                               uint32_t* row_count, const uint32_t* row_off,
                                                               uint4* table, uint32_t* nchunks, uint32_t chunk,
-                                                              uint32_t cspan, uint32_t stride)
+                                                              uint32_t cspan, uint32_t stride, uint32_t cap,
+                                                              uint32_t* overflow)
 {
     extern __shared__ uint32_t rs[];            // kRowWaves x stride (>= the longest row's columns + 1)
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kRowWaves + w;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (WRITE) *nchunks = row_off[F.rows];
-        else row_count[F.rows] = 0u;            // the scan's total lands in row_off[rows]
+        if (WRITE) {
+            const uint32_t total = row_off[F.rows];
+            *nchunks = min(total, cap);
+            if (total > cap) *overflow = total;
+        } else {
+            row_count[F.rows] = 0u;             // the scan's total lands in row_off[rows]
+        }
     }
     if (r >= F.rows) return;                    // whole waves only: no barrier follows
     const uint32_t f = F.row_flock ? F.row_flock[r] : 0u;
@@ -218,7 +229,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_pass(Flocks F, const ui
             while (cs[xa + 1] <= t0) ++xa;          // cell of boid t0
             uint32_t t1 = min(hi, t0 + chunk);
             if (xa + cspan < x1) t1 = min(t1, cs[xa + cspan]);
-            if (WRITE) table[k] = make_uint4(rl, t0, t1, f);
+            if (WRITE && k < cap) table[k] = make_uint4(rl, t0, t1, f);
             ++k;
             t0 = t1;
         }
@@ -346,8 +357,8 @@ hipError_t launch_scan(uint32_t* count, uint32_t* start, uint32_t m, uint32_t* p
 }
 
 hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uint32_t* row_off,
-                       uint32_t* partials, uint4* table, uint32_t* nchunks, int chunk, int cspan, int max_cols,
-                       hipStream_t stream)
+                       uint32_t* partials, uint4* table, uint32_t* nchunks, uint32_t cap, uint32_t* overflow,
+                       int chunk, int cspan, int max_cols, hipStream_t stream)
 {
     if (F.count == 0 || chunk <= 0 || F.rows == 0 || cspan < 1 || cspan > kChunkSpanMax || max_cols < 1 ||
         max_cols > kSubXMax * kMaxCells)
@@ -356,11 +367,13 @@ hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uin
     const uint32_t stride = (uint32_t)max_cols + 1;
     const size_t dyn = sizeof(uint32_t) * kRowWaves * stride;
     hipLaunchKernelGGL(k_rows_pass<false>, grid, dim3(64 * kRowWaves), dyn, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride);
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride, cap,
+                       overflow);
     hipError_t st = launch_scan(row_count, row_off, F.rows + 1, partials, stream);
     if (st != hipSuccess) return st;
     hipLaunchKernelGGL(k_rows_pass<true>, grid, dim3(64 * kRowWaves), dyn, stream, F, start, row_count,
-                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride);
+                       (const uint32_t*)row_off, table, nchunks, (uint32_t)chunk, (uint32_t)cspan, stride, cap,
+                       overflow);
     return hipGetLastError();
 }
 

@@ -264,6 +264,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const uint32_t f = __builtin_amdgcn_readfirstlane(ch.w);
     const DevParams P = FL.fl[f].P;
     const uint32_t cbase = FL.fl[f].cell_base;
+    const uint32_t idbase = DIGEST ? FL.fl[f].boid_base : 0u;   // digests hash flock-local ids
     const int G = P.G;
     const int sx = P.sx;                              // x columns per cell edge
     const uint32_t Gx = (uint32_t)(sx * G);           // x columns of a row
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         if (in_c | in_s | in_a)
             atomicAdd(&acc_n[o], (uint64_t)in_c | ((uint64_t)in_s << 21) | ((uint64_t)in_a << 42));
         if (DIGEST) {
-            const uint64_t h = mix64(in.id[r.j]);
+            const uint64_t h = mix64(in.id[r.j] - idbase);
             if (in_c) atomicAdd(&acc_h[o], h);
             if (in_s) atomicAdd(&acc_h[NB + o], h);
             if (in_a) atomicAdd(&acc_h[2 * NB + o], h);
@@ -477,6 +478,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const uint32_t f = __builtin_amdgcn_readfirstlane(ch.w);
     const DevParams P = FL.fl[f].P;
     const uint32_t cbase = FL.fl[f].cell_base;
+    const uint32_t idbase = DIGEST ? FL.fl[f].boid_base : 0u;   // digests hash flock-local ids
     const int G = P.G;
     const int sx = P.sx;                              // x columns per cell edge
     const uint32_t Gx = (uint32_t)(sx * G);           // x columns of a row
@@ -562,7 +564,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         if (in_c | in_s | in_a)
             atomicAdd(&acc_n[o], (uint64_t)in_c | ((uint64_t)in_s << 21) | ((uint64_t)in_a << 42));
         if (DIGEST) {
-            const uint64_t h = mix64(in.id[j]);
+            const uint64_t h = mix64(in.id[j] - idbase);
             if (in_c) atomicAdd(&acc_h[o], h);
             if (in_s) atomicAdd(&acc_h[NB + o], h);
             if (in_a) atomicAdd(&acc_h[2 * NB + o], h);

@@ -23,8 +23,9 @@ MODE_ALLPAIRS_F32REF = 0
 MODE_ALLPAIRS_F64 = 1
 MODE_GRID_F64 = 2
 MODE_GRID_MIXED = 3
+MODE_GRID_F32REF = 4
 MODE_NAMES = {MODE_ALLPAIRS_F32REF: "allpairs_f32ref", MODE_ALLPAIRS_F64: "allpairs_f64",
-              MODE_GRID_F64: "grid_f64", MODE_GRID_MIXED: "grid_mixed"}
+              MODE_GRID_F64: "grid_f64", MODE_GRID_MIXED: "grid_mixed", MODE_GRID_F32REF: "grid_f32ref"}
 
 INIT_TEST = 0
 INIT_RANDOM_UNIFORM = 1
@@ -81,7 +82,14 @@ EXPORTS = ["bs_create", "bs_destroy", "bs_last_error", "bs_generate_init", "bs_r
            "bs_get_state_f64", "bs_get_forces_f32", "bs_get_forces_f64", "bs_get_digest",
            "bs_get_info", "bs_get_timing", "bs_reset_timing", "bs_debug_thresholds",
            "bs_debug_rechecks", "bs_debug_copy", "bs_metrics", "bs_version", "bs_create_batch",
-           "bs_get_flock_count", "bs_get_flock_info", "bs_metrics_flock"]
+           "bs_get_flock_count", "bs_get_flock_info", "bs_metrics_flock", "bs_debug_grid"]
+
+
+class GridInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("ncells", ctypes.c_uint32), ("nchunks", ctypes.c_uint32),
+                ("max_chunks", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
+                ("x_columns_per_cell", ctypes.c_int32), ("chunk_boids", ctypes.c_int32),
+                ("chunk_span", ctypes.c_int32)]
 
 
 class Metrics(ctypes.Structure):
@@ -130,6 +138,8 @@ def lib():
         "bs_get_flock_count": (c_int, [vp, P(ctypes.c_uint32)]),
         "bs_get_flock_info": (c_int, [vp, ctypes.c_uint32, P(FlockInfo)]),
         "bs_metrics_flock": (c_int, [vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
+        "bs_debug_grid": (c_int, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32),
+                                  P(ctypes.c_uint32), P(GridInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -293,6 +303,20 @@ class Engine:
                         "bs_metrics_flock")
         return m.as_dict()
 
+    def grid(self):
+        """The grid of the last sort: (info, sorted cell keys, sorted ids, cell
+        starts, chunk table as nchunks x 4)."""
+        info = GridInfo()
+        self._check(lib().bs_debug_grid(self._h, None, None, None, None, ctypes.byref(info)), "bs_debug_grid")
+        u32 = ctypes.c_uint32
+        key = np.zeros(info.n, np.uint32)
+        ids = np.zeros(info.n, np.uint32)
+        start = np.zeros(info.ncells + 1, np.uint32)
+        table = np.zeros((max(info.max_chunks, 1), 4), np.uint32)
+        self._check(lib().bs_debug_grid(self._h, _p(key, u32), _p(ids, u32), _p(start, u32), _p(table, u32),
+                                        ctypes.byref(info)), "bs_debug_grid")
+        return info, key, ids, start, table[:min(info.nchunks, info.max_chunks)].copy()
+
     def flock_count(self) -> int:
         c = ctypes.c_uint32()
         self._check(lib().bs_get_flock_count(self._h, ctypes.byref(c)), "bs_get_flock_count")
@@ -343,7 +367,7 @@ class BoidSimulation:
     def init(self):
         """src/boid_simulation.cpp:79-137.  The all-pairs engine never seeds
         drand48; the grid engine calls srand48(seed) (src/boid_simulation_grid.cpp:98)."""
-        seeded = self.mode in (MODE_GRID_F64, MODE_GRID_MIXED)
+        seeded = self.mode in (MODE_GRID_F64, MODE_GRID_MIXED, MODE_GRID_F32REF)
         F = self.field_size
         vmax, vmin = self.velocity
         if self.initialization_type in ("test", "random_uniform"):

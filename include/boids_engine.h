@@ -51,7 +51,14 @@ enum {
      * pair tests on exact 32-bit lattice differences, FP64 recheck of pairs
      * inside a guard band around every threshold, force sums of
      * integer-valued terms in FP64 (exact, so independent of order). */
-    BS_MODE_GRID_MIXED = 3
+    BS_MODE_GRID_MIXED = 3,
+    /* float state, the reference's arithmetic, uniform grid of cells with
+     * edge >= the largest radius; every owner visits the in-range boids of
+     * its 27 cells in ascending id, so the float sums are the reference's:
+     * bit-identical to boidsim_grid (src/boid_simulation_grid.cpp:165-293),
+     * which equals boidsim.  Falls back to ALLPAIRS_F32REF (the same bits)
+     * below 3 cells per axis. */
+    BS_MODE_GRID_F32REF = 4
 };
 
 /* Init kinds (src/boid_simulation.cpp:79-137 for the first two). */
@@ -180,7 +187,9 @@ int bs_get_forces_f64(bs_engine* e, double* f9);
 /* Neighbour digests of the last update: n*4 uint64 per boid,
  * [0] = n_coh | n_sep << 21 | n_ali << 42, [1..3] = sum over neighbours j
  * of mix64(j) for cohesion, separation, alignment (mix64 = splitmix64
- * finaliser of j + 0x9E3779B97F4A7C15).  Needs BS_FLAG_DIGEST. */
+ * finaliser of j + 0x9E3779B97F4A7C15).  j is the neighbour's id within its
+ * flock (in a batch, id - boid_base), so a flock's digests equal those of
+ * the same flock in an engine of its own.  Needs BS_FLAG_DIGEST. */
 int bs_get_digest(bs_engine* e, uint64_t* d4);
 
 int bs_get_info(const bs_engine* e, bs_info* out);
@@ -198,6 +207,26 @@ int bs_debug_thresholds(const bs_params* p, double* out6);
 /* Mixed mode: number of candidate pairs decided by the FP64 recheck since
  * creation (guard-band hits).  0 for other modes. */
 int bs_debug_rechecks(bs_engine* e, uint64_t* out);
+
+/* Grid built by the last counting sort (grid modes), for exactness tests of
+ * the build that replaces the reference's cell-list registration
+ * (update_list_grid / clear_list_grid, src/boid_simulation_grid.cpp:383-451).
+ * key_sorted[n]: global cell key of every sorted slot; id_sorted[n]: the
+ * boid id in that slot; start[ncells + 1]: first slot of every cell;
+ * table4[4 * nchunks]: the neighbour kernels' chunk table, entries (y,z row
+ * within the flock, first slot, end slot, flock).  Any array may be NULL;
+ * info is filled in every case.  Cell keys: x column (x_columns_per_cell
+ * columns per cell edge) + columns * (y cell + G * z cell), plus the flock's
+ * cell_base. */
+typedef struct {
+    uint32_t n, ncells, nchunks, max_chunks;
+    uint32_t overflow;           /* chunk count beyond max_chunks (0: none; else results are invalid) */
+    int32_t x_columns_per_cell;  /* x columns per cell edge (1 or 2) */
+    int32_t chunk_boids;         /* most boids per chunk */
+    int32_t chunk_span;          /* most x columns a chunk covers */
+} bs_grid_info;
+int bs_debug_grid(bs_engine* e, uint32_t* key_sorted, uint32_t* id_sorted, uint32_t* start, uint32_t* table4,
+                  bs_grid_info* info);
 
 /* Flock metrics of the current state, for long-run statistical comparison.
  * Periodic distance as tools/calc_distance.pyx:6-14.  polarisation =

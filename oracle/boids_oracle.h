@@ -70,6 +70,17 @@ int bo_step_f32(const bo_params* p, uint32_t n, float* pos, float* vel,
 int bo_step_f64(const bo_params* p, uint32_t n, double* pos, double* vel,
                 double* forces, uint64_t* digest, int enumeration, int nthreads);
 
+/* CPU-baseline timing helper (bench.py's cpu_baseline leg): the parts of
+ * one cell-enumeration update, timed without changing the state --
+ * seconds[0] = cell-list build, seconds[1] = phase A of owners 0, stride,
+ * 2 stride, ..., seconds[2] = phase B of all boids (on a scratch copy).
+ * Returns the number of sampled owners, or -1 (bad arguments, or a field
+ * with fewer than 3 cells per axis). */
+long bo_time_step_sample_f32(const bo_params* p, uint32_t n, const float* pos, const float* vel,
+                             uint32_t stride, int nthreads, double* seconds);
+long bo_time_step_sample_f64(const bo_params* p, uint32_t n, const double* pos, const double* vel,
+                             uint32_t stride, int nthreads, double* seconds);
+
 /* Cells per axis the oracle's cell enumeration uses (0 => all-pairs). */
 int bo_cells_per_axis(const bo_params* p);
 

@@ -51,6 +51,11 @@ def lib():
             f.restype = ctypes.c_int
             f.argtypes = [P(Params), ctypes.c_uint32, P(real), P(real), P(real),
                           P(ctypes.c_uint64), ctypes.c_int, ctypes.c_int]
+        for name, real in (("bo_time_step_sample_f32", ctypes.c_float), ("bo_time_step_sample_f64", ctypes.c_double)):
+            f = getattr(L, name)
+            f.restype = ctypes.c_long
+            f.argtypes = [P(Params), ctypes.c_uint32, P(real), P(real), ctypes.c_uint32, ctypes.c_int,
+                          P(ctypes.c_double)]
         L.bo_cells_per_axis.restype = ctypes.c_int
         L.bo_cells_per_axis.argtypes = [P(Params)]
         L.bo_mix64.restype = ctypes.c_uint64
@@ -112,6 +117,20 @@ def step(p: Params, pos, vel, *, forces=False, digest=False, enumeration=ENUM_AL
     if rc != 0:
         raise RuntimeError("oracle step failed")
     return f, d
+
+
+def time_step_sample(p: Params, pos, vel, stride, nthreads=0):
+    """Timed parts of one cell-enumeration update without changing the state:
+    (sampled owners, build s, phase A of every stride-th owner s, phase B s)."""
+    pos = np.ascontiguousarray(pos)
+    vel = np.ascontiguousarray(vel, dtype=pos.dtype)
+    ct = ctypes.c_float if pos.dtype == np.float32 else ctypes.c_double
+    fn = lib().bo_time_step_sample_f32 if pos.dtype == np.float32 else lib().bo_time_step_sample_f64
+    sec = np.zeros(3, np.float64)
+    k = fn(ctypes.byref(p), pos.shape[0], _ptr(pos, ct), _ptr(vel, ct), stride, nthreads, _ptr(sec, ctypes.c_double))
+    if k < 0:
+        raise RuntimeError("oracle timing sample failed")
+    return int(k), float(sec[0]), float(sec[1]), float(sec[2])
 
 
 def cells_per_axis(p: Params) -> int:

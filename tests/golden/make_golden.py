@@ -7,7 +7,8 @@ BoidSimulation API and stores inputs and expected outputs as .npz data.
 Needs /root/reference (this container only); the fixtures it writes are
 committed and travel without the reference.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py               # every fixture
+    python tests/golden/make_golden.py grid_testini  # only ref_grid_testini_n1000.npz
 """
 from __future__ import annotations
 
@@ -61,8 +62,25 @@ def rules_array(F, r):
     return np.array([F, *r["sep"], *r["ali"], *r["coh"], r["vmax"], r["vmin"]], np.float64)
 
 
+def grid_testini():
+    """6. boidsim_grid with settings/test.ini: the whole .ptcl file (positions
+    and velocities) as a digest.  The grid engine seeds srand48(12345) and
+    sizes GRID_NUM = 4 here (src/boid_simulation_grid.cpp:80-89)."""
+    n, T, F = 1000, 30, 1.0
+    d = run_ref("ref_grid", n, T, F, TESTINI, "test", 12345, list(range(T + 1)))
+    pos = {t: d[t][0] for t in d}
+    vel = {t: d[t][1] for t in d}
+    save("ref_grid_testini_n1000.npz", rules=rules_array(F, TESTINI),
+         pos0=d[0][0], vel0=d[0][1], pos30=d[30][0], vel30=d[30][1],
+         ptcl_sha256=np.frombuffer(hashlib.sha256(ptcl_bytes(pos, n, T, F)).digest(), np.uint8),
+         vptcl_sha256=np.frombuffer(hashlib.sha256(ptcl_bytes(vel, n, T, F)).digest(), np.uint8))
+
+
 def main():
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile.ref")], check=True)
+    if sys.argv[1:] == ["grid_testini"]:      # one fixture only (the others stay byte-identical)
+        grid_testini()
+        return
 
     # 1. boidsim, random_uniform init (drand48 never seeded), cfg0 rules
     n, T, F = 1024, 100, 1.0
@@ -113,6 +131,8 @@ def main():
     d = run_ref("ref_grid", n, T, F, EXGRID, "random_uniform", 12345, [0, 20])
     save("ref_grid_distonly_n2048.npz", rules=rules_array(F, EXGRID),
          pos0=d[0][0], vel0=d[0][1], pos20=d[20][0], vel20=d[20][1])
+
+    grid_testini()
 
 
 if __name__ == "__main__":

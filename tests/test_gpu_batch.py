@@ -92,7 +92,7 @@ def test_batch_equals_single_engines(mode):
             p, v = e.state_f64()
             assert np.array_equal(p, pb[k]) and np.array_equal(v, vb[k]), (t, k)
             assert np.array_equal(e.forces_f64(), fb[k]), (t, k)
-            assert np.array_equal(e.digest()[:, 0], db[k][:, 0]), (t, k)   # neighbour counts per rule
+            assert np.array_equal(e.digest(), db[k]), (t, k)   # neighbour sets (flock-local ids)
     for e in singles:
         e.close()
     sw.close()
@@ -166,3 +166,64 @@ def test_sweep_front_end_matches_boidsim_grid(tmp_path):
         a = hashlib.sha256(open(out, "rb").read()).digest()
         b = hashlib.sha256(open(tmp_path / "sweep" / (j.name + ".ptcl"), "rb").read()).digest()
         assert a == b, j.name
+
+
+@pytest.mark.parametrize("mode", [E.MODE_GRID_MIXED, E.MODE_GRID_F64])
+def test_cfg3_full_sweep_against_oracle(mode):
+    # The configured sweep itself: 512 flocks, N log-uniform in [2^10, 2^16]
+    # (8.7 M boids), per-flock fields, vmax ~ U[0.002, 0.01], vmin = vmax / 5,
+    # radii x U[0.7, 1.3], seed 3 (settings/parameter_search_vel_2/gen.py:32-42
+    # and tools/gen_jobscript.py:37-65 are the reference's form of a sweep:
+    # one job per parameter set).  One batched engine steps every flock; the
+    # extreme flocks (smallest and largest N, smallest and largest radius
+    # scale and vmax) are checked against the FP64 oracle on their own rules
+    # every step: neighbour sets exact, per-step error <= 1e-12 (FP64) or
+    # 1e-6 (mixed) of the field and of the flock's vmax.  Every flock stays in
+    # its field with vmin <= |v| <= vmax.
+    from oracle import oracle as O
+    jobs = S.generated_sweep(512, 3)
+    scale = [j.coh[0] / 0.08 for j in jobs]
+    picks = sorted({int(np.argmin([j.n for j in jobs])), int(np.argmax([j.n for j in jobs])),
+                    int(np.argmin(scale)), int(np.argmax(scale)),
+                    int(np.argmin([j.vmax for j in jobs])), int(np.argmax([j.vmax for j in jobs]))})
+    say(f"cfg3 {E.MODE_NAMES[mode]}: {sum(j.n for j in jobs)} boids in {len(jobs)} flocks; oracle on flocks {picks}: "
+        + ", ".join(f"n={jobs[k].n} s={scale[k]:.3f} vmax={jobs[k].vmax:.4f}" for k in picks))
+    e = E.Engine([j.params(mode, E.FLAG_DIGEST) for j in jobs])
+    assert e.flock_count() == 512
+    e.set_state(xyzv6=np.concatenate([S.initial_state(j) for j in jobs]))
+    bases = [e.flock_info(k).boid_base for k in range(512)]
+    tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+    gp, gv = e.state_f64()
+    worst = [0.0, 0.0]
+    for t in range(3):
+        want = {}
+        for k in picks:
+            j, b = jobs[k], bases[k]
+            P = O.params(j.field, j.sep, j.ali, j.coh, j.vmax, j.vmin)
+            p, v = gp[b:b + j.n].copy(), gv[b:b + j.n].copy()
+            _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+            want[k] = (p, v, d)
+        e.step(1)
+        gd = e.digest()
+        gp, gv = e.state_f64()
+        for k in picks:
+            j, b = jobs[k], bases[k]
+            p, v, d = want[k]
+            bad = np.nonzero((gd[b:b + j.n] != d).any(1))[0]   # a batch's digests hash flock-local ids
+            assert bad.size == 0, (t, k, f"{bad.size} boids with different neighbour sets")
+            dp = gp[b:b + j.n] - p
+            dp -= j.field * np.round(dp / j.field)
+            ep = float(np.abs(dp).max() / j.field)
+            ev = float(np.abs(gv[b:b + j.n] - v).max() / j.vmax)
+            worst = [max(worst[0], ep), max(worst[1], ev)]
+            assert ep <= tol and ev <= tol, (t, k, ep, ev)
+        for k, j in enumerate(jobs):
+            b = bases[k]
+            p, v = gp[b:b + j.n], gv[b:b + j.n]
+            s = np.linalg.norm(v, axis=1)
+            assert np.isfinite(p).all() and (p >= 0).all() and (p <= j.field).all(), (t, k)
+            assert (s >= j.vmin * (1 - 1e-12)).all() and (s <= j.vmax * (1 + 1e-12)).all(), (t, k)
+        say(f"  step {t}: ok")
+    say(f"cfg3 {E.MODE_NAMES[mode]}: max per-step error {worst[0]:.3g} F, {worst[1]:.3g} vmax; "
+        f"rechecks {e.rechecks()}")
+    e.close()

@@ -1,8 +1,9 @@
 """The drop-in executables on the GPU: boidsim reproduces the reference
 boidsim's .ptcl output byte for byte (settings/test.ini golden digests), the
 split/resume invariant of test/compare_split.sh holds with the executables,
---force-data-output writes the reference's 12-float records, and
-boidsim_grid tracks the FP64 oracle."""
+--force-data-output writes the reference's 12-float records, boidsim_grid
+reproduces the reference boidsim_grid's output byte for byte, and its FP64
+mode tracks the FP64 oracle."""
 import hashlib
 import os
 import subprocess
@@ -68,9 +69,35 @@ def test_boidsim_force_output(tmp_path):
         assert np.array_equal(fr[t, :, :3], p) and np.array_equal(fr[t, :, 3:], frc * w), t
 
 
-def test_boidsim_grid_tracks_fp64_oracle(tmp_path):
+def test_boidsim_grid_matches_reference_output(golden_dir, tmp_path):
+    # boidsim_grid runs the float grid mode: the reference boidsim_grid's
+    # .ptcl and velocity files for settings/test.ini, byte for byte (its own
+    # init: srand48(12345) then the "test" kind; GRID_NUM 4)
+    g = np.load(os.path.join(golden_dir, "ref_grid_testini_n1000.npz"))
+    out = run("boidsim_grid", "-s", TESTINI, "-o", "g.ptcl", "--velocity-output", "gv.ptcl", cwd=tmp_path)
+    assert "grid num:" in out and "simulation finished." in out
+    assert sha(tmp_path / "g.ptcl") == g["ptcl_sha256"].tobytes()
+    assert sha(tmp_path / "gv.ptcl") == g["vptcl_sha256"].tobytes()
+
+
+def test_boidsim_grid_split_resume_equals_continuous(tmp_path):
+    # test/compare_split.sh with the grid executable (the grid is rebuilt
+    # after a file init)
+    run("boidsim_grid", "-s", TESTINI, "-T", "30", "-o", "full.ptcl", cwd=tmp_path)
+    run("boidsim_grid", "-s", TESTINI, "-T", "15", "-o", "a.ptcl", "--velocity-output", "av.ptcl", cwd=tmp_path)
+    ptcl.generate_initialization_file(tmp_path / "a.ptcl", tmp_path / "av.ptcl", 14, tmp_path / "init.bin")
+    run("boidsim_grid", "-s", TESTINI, "-T", "16", "-o", "b.ptcl", "--initialization", str(tmp_path / "init.bin"),
+        cwd=tmp_path)
+    ptcl.split(tmp_path / "b.ptcl", tmp_path / "tmp.ptcl", 1, 15)
+    ptcl.merge([tmp_path / "a.ptcl", tmp_path / "tmp.ptcl"], tmp_path / "merged.ptcl")
+    assert sha(tmp_path / "merged.ptcl") == sha(tmp_path / "full.ptcl")
+
+
+def test_boidsim_grid_fp64_mode_tracks_fp64_oracle(tmp_path):
+    # BOIDS_ENGINE_MODE=grid_f64 runs the FP64 grid path behind the same
+    # executable: it follows the FP64 oracle from the grid engine's init
     out = run("boidsim_grid", "-s", TESTINI, "-T", "20", "-N", "4000", "-F", "1.2", "-o", "g.ptcl",
-              "--velocity-output", "gv.ptcl", cwd=tmp_path)
+              "--velocity-output", "gv.ptcl", cwd=tmp_path, env={"BOIDS_ENGINE_MODE": "grid_f64"})
     assert "grid num:" in out
     fr = ptcl.PtclFile(tmp_path / "g.ptcl").frames()
     P = O.params(1.2, *TESTINI_RULES)

@@ -109,6 +109,61 @@ def test_f32ref_testini_ptcl(golden_dir):
     assert hp.digest() == g["ptcl_sha256"].tobytes() and hv.digest() == g["vptcl_sha256"].tobytes()
 
 
+# ------------------------------------------ float grid mode vs reference ---
+@pytest.mark.parametrize("name,steps", [("ref_grid_ru_seed1_n2048.npz", 50), ("ref_grid_distonly_n2048.npz", 20),
+                                        ("ref_grid_testini_n1000.npz", 30)])
+def test_f32_grid_matches_reference_grid(golden_dir, name, steps):
+    # BS_MODE_GRID_F32REF is boidsim_grid's arithmetic: bit-identical to the
+    # reference's own grid engine on its own fixtures
+    g = np.load(os.path.join(golden_dir, name))
+    F, rules = rules_of(g["rules"])
+    n = g["pos0"].shape[0]
+    e = make(n, F, rules, E.MODE_GRID_F32REF)
+    assert e.info().mode == E.MODE_GRID_F32REF and e.info().cells_per_axis >= 3
+    e.set_state(xyzv6=np.concatenate([g["pos0"], g["vel0"]], 1))
+    e.step(steps)
+    p, v = e.state_f32()
+    assert np.array_equal(p, g[f"pos{steps}"]) and np.array_equal(v, g[f"vel{steps}"])
+
+
+def test_f32_grid_file_init_forces(golden_dir):
+    g = np.load(os.path.join(golden_dir, "ref_allpairs_file_n1024.npz"))
+    F, rules = rules_of(g["rules"])
+    e = make(1024, F, rules, E.MODE_GRID_F32REF, E.FLAG_FORCES)
+    assert e.info().mode == E.MODE_GRID_F32REF
+    e.set_state(xyzv6=g["init"])
+    for t in range(1, 101):
+        e.step(1)
+        if t in (1, 10, 100):
+            p, v = e.state_f32()
+            assert np.array_equal(p, g[f"pos{t}"]) and np.array_equal(v, g[f"vel{t}"]), t
+        if t in (1, 100):
+            assert np.array_equal(e.forces_f32() * wvec(rules), g[f"force{t}"]), t
+
+
+@pytest.mark.parametrize("n,F,cells", [(16384, 1.0, 0), (3000, 0.25, 3), (20000, 1.6, 0)])
+def test_f32_grid_equals_allpairs(n, F, cells):
+    # the float grid equals the float all-pairs step bit for bit over a long
+    # run (chaotic: any difference in a sum's order would grow), with its
+    # neighbour digests and forces; also on the smallest grid (3 cells per axis)
+    st = iso_state(n, F, 3)
+    a = make(n, F, CFG0, E.MODE_ALLPAIRS_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
+    b = make(n, F, CFG0, E.MODE_GRID_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES, cells=cells)
+    assert b.info().mode == E.MODE_GRID_F32REF and (cells == 0 or b.info().cells_per_axis == cells)
+    a.set_state(xyzv6=st)
+    b.set_state(xyzv6=st)
+    for t in range(4):
+        steps = 1 if t < 3 else 150
+        a.step(steps)
+        b.step(steps)
+        pa, va = a.state_f32()
+        pb, vb = b.state_f32()
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb), t
+        assert np.array_equal(a.digest(), b.digest()), t
+        assert np.array_equal(a.forces_f32(), b.forces_f32()), t
+    say(f"float grid == float all-pairs over 153 updates, N={n}, {b.info().cells_per_axis} cells per axis")
+
+
 # ------------------------------------------------------- FP64 all-pairs ---
 def test_f64_allpairs_bit_exact_with_oracle():
     n, F = 1024, 1.0
@@ -378,7 +433,8 @@ def test_cfg4_full_size_mixed_equals_fp64_neighbour_sets():
 
 
 # ---------------------------------------------------------- edge cases ---
-@pytest.mark.parametrize("mode", [E.MODE_ALLPAIRS_F32REF, E.MODE_ALLPAIRS_F64, E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+@pytest.mark.parametrize("mode", [E.MODE_ALLPAIRS_F32REF, E.MODE_ALLPAIRS_F64, E.MODE_GRID_F64, E.MODE_GRID_MIXED,
+                                  E.MODE_GRID_F32REF])
 def test_tiny_and_degenerate_inputs(mode):
     F = 1.0
     # n = 0
@@ -398,7 +454,7 @@ def test_tiny_and_degenerate_inputs(mode):
     e.set_state(xyzv6=st)
     e.step(1)
     P = O.params(F, **CFG0)
-    if mode == E.MODE_ALLPAIRS_F32REF:
+    if mode in (E.MODE_ALLPAIRS_F32REF, E.MODE_GRID_F32REF):
         p, v = st[:, :3].copy(), st[:, 3:].copy()
         _, d = O.step(P, p, v, digest=True)
         gp, gv = e.state_f32()

@@ -129,6 +129,23 @@ def test_f32_testini_ptcl(golden_dir):
     assert np.array_equal(p, g["pos30"]) and np.array_equal(v, g["vel30"])
 
 
+@pytest.mark.parametrize("enum", [O.ENUM_ALLPAIRS, O.ENUM_CELLS])
+def test_f32_grid_testini_ptcl(golden_dir, enum):
+    # the whole boidsim_grid output for settings/test.ini (its own init: the
+    # "test" kind after srand48(12345)), byte for byte
+    g = load(golden_dir, "ref_grid_testini_n1000.npz")
+    P = params_from(g["rules"])
+    p, v = O.init_reference("test", 1000, 1.0, 0.003, 0.001, seed=12345)
+    assert np.array_equal(p, g["pos0"]) and np.array_equal(v, g["vel0"])
+    frames, vframes = [], []
+    for _ in range(30):
+        frames.append(p.copy()); vframes.append(v.copy())
+        O.step(P, p, v, enumeration=enum)
+    assert hashlib.sha256(ptcl(frames, 1000, 30, 1.0)).digest() == g["ptcl_sha256"].tobytes()
+    assert hashlib.sha256(ptcl(vframes, 1000, 30, 1.0)).digest() == g["vptcl_sha256"].tobytes()
+    assert np.array_equal(p, g["pos30"]) and np.array_equal(v, g["vel30"])
+
+
 # --------------------------------------------------- FP64: closed forms ---
 def f64(pos, vel):
     return np.array(pos, np.float64).reshape(-1, 3), np.array(vel, np.float64).reshape(-1, 3)
