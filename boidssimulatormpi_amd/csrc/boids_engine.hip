@@ -325,6 +325,9 @@ static int derive_thresholds(const bs_params* p, DevParams* P, std::string* why)
         P->c_top[k] = P->cone[k] ? round_down_f(1.0 - beta_c) : INFINITY;
     }
     P->band_k = 8.0f;   // >= 2*sqrt(3) lattice units of displacement error, x2 margin
+    P->same_cone = 1;
+    for (int k = 1; k < 3; ++k)
+        if (!(P->c_lo[k] == P->c_lo[0] && P->c_hi[k] == P->c_hi[0] && P->c_top[k] == P->c_top[0])) P->same_cone = 0;
     // Coarse filter on the top 16 lattice bits (unit F/2^16): a coarse
     // difference is off by at most one unit per axis, so every pair with
     // d < max(r) + beta_d has coarse squared distance < (r16 + 2)^2.

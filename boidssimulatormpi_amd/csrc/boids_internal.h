@@ -62,6 +62,7 @@ struct DevParams {
                                // (a rule without view cone: -inf, the cosine decides nothing)
     float c_top[3];            // c32 >= c_top: too close to 1, recheck (no view cone: +inf)
     float band_k;              // k: cosine band growth with 1/d (lattice units)
+    int32_t same_cone;         // 1: c_lo, c_hi and c_top agree for the three rules (one shared cone test)
     int32_t coarse_thr;        // squared distance bound on the top 16 lattice bits
     // velocity quantum of the mixed lattice records: a power of two with
     // |v| / vq < 2^31 for |v| <= vmax, so quantised velocities are integers

@@ -86,13 +86,20 @@ __device__ __forceinline__ uint2 coarse_of(uint32_t lx, uint32_t ly, uint32_t lz
     return make_uint2((lx >> 16) | (ly & 0xFFFF0000u), lz >> 16);
 }
 
-// Squared coarse distance (packed int16 minimum-image differences; the
-// dot products saturate instead of wrapping).
+// Squared coarse distance (int16 minimum-image differences, the x|y pair
+// packed; the dot product saturates instead of wrapping).  z is the low
+// half of q.y (the high half holds the tile entry): its difference is taken
+// mod 2^16 and squared as a sign-extended 16-bit value, dz^2 <= 2^30.
 __device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, v2s my_z)
 {
     const v2s dxy = __builtin_bit_cast(v2s, q.x) - my_xy;
-    const v2s dzz = __builtin_bit_cast(v2s, q.y) - my_z;
+#if BS_P1_Z16
+    const int32_t dz = (int16_t)(uint16_t)(q.y - (uint32_t)(uint16_t)my_z.x);
+    return __builtin_amdgcn_sdot2(dxy, dxy, dz * dz, true);
+#else
+    const v2s dzz = __builtin_bit_cast(v2s, q.y & 0xFFFFu) - my_z;
     return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dzz, dzz, 0, true), true);
+#endif
 }
 
 // Stages tile positions [base, base + here); each thread copies one run of
@@ -181,11 +188,35 @@ template <uint32_t R> struct ListCfg {
 // by the evaluation.  flush(tail) evaluates the list's full rounds and
 // returns the number of entries it leaves at the front.  Returns the final
 // tail.  Every lane of the wave takes part.
-template <uint32_t kDrain, typename FLUSH>
+#ifndef BS_P1_Z16
+#define BS_P1_Z16 0
+#endif
+#ifndef BS_P1_TRASH
+#define BS_P1_TRASH 0
+#endif
+#ifndef BS_LIST_POS
+#define BS_LIST_POS 0
+#endif
+#ifndef BS_P2_SEPW
+#define BS_P2_SEPW 0
+#endif
+#ifndef BS_P2_CONE1
+#define BS_P2_CONE1 0
+#endif
+// List entry of a hit at tile position k (e = k << 6 | lane): with
+// BS_LIST_POS the entry is e alone and phase 2 reads the tile entry (cell)
+// from the tile; otherwise the tile entry rides in the top 10 bits.
+#if BS_LIST_POS
+#define BS_LIST_VAL(a, e) (e)
+#else
+#define BS_LIST_VAL(a, e) (((a).y & 0xFFC00000u) | (e))
+#endif
+
+template <uint32_t kDrain, uint32_t kTrash, typename FLUSH>
 __device__ __forceinline__ uint32_t filter_window(const uint2* tco, uint32_t* list, uint32_t k, uint32_t k1,
                                                   uint32_t lane, v2s my_xy, v2s my_z, int32_t thr, FLUSH&& flush)
 {
-    auto near = [&](uint2 q) { return coarse_d2(make_uint2(q.x, q.y & 0xFFFFu), my_xy, my_z) <= thr; };
+    auto near = [&](uint2 q) { return coarse_d2(q, my_xy, my_z) <= thr; };
     uint32_t tail = 0;
     uint2 a0 = tco[k], a1 = tco[k + 1], a2 = tco[k + 2], a3 = tco[k + 3];
     while (wave_mask(k < k1) != 0) {
@@ -202,11 +233,19 @@ __device__ __forceinline__ uint32_t filter_window(const uint2* tco, uint32_t* li
         const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
         const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
         const uint32_t e = (k << 6) | lane;
+#if BS_P1_TRASH
+        // no exec-mask changes: a miss is stored to its lane's slot past the list
+        list[h0 ? tail + lanes_below(m0) : kTrash + lane] = BS_LIST_VAL(a0, e);
+        list[h1 ? t1 + lanes_below(m1) : kTrash + lane] = BS_LIST_VAL(a1, e + (1u << 6));
+        list[h2 ? t2 + lanes_below(m2) : kTrash + lane] = BS_LIST_VAL(a2, e + (2u << 6));
+        list[h3 ? t3 + lanes_below(m3) : kTrash + lane] = BS_LIST_VAL(a3, e + (3u << 6));
+#else
         // the wave-uniform list offsets go into the (scalar) base address
-        if (h0) (list + tail)[lanes_below(m0)] = (a0.y & 0xFFC00000u) | e;
-        if (h1) (list + t1)[lanes_below(m1)] = (a1.y & 0xFFC00000u) | (e + (1u << 6));
-        if (h2) (list + t2)[lanes_below(m2)] = (a2.y & 0xFFC00000u) | (e + (2u << 6));
-        if (h3) (list + t3)[lanes_below(m3)] = (a3.y & 0xFFC00000u) | (e + (3u << 6));
+        if (h0) (list + tail)[lanes_below(m0)] = BS_LIST_VAL(a0, e);
+        if (h1) (list + t1)[lanes_below(m1)] = BS_LIST_VAL(a1, e + (1u << 6));
+        if (h2) (list + t2)[lanes_below(m2)] = BS_LIST_VAL(a2, e + (2u << 6));
+        if (h3) (list + t3)[lanes_below(m3)] = BS_LIST_VAL(a3, e + (3u << 6));
+#endif
         tail = t3 + __popcll(m3);
         if (tail >= kDrain) tail = flush(tail);
         k = kn;
@@ -248,7 +287,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     } sh;
     auto& tco = sh.tco;
     auto& acc = sh.acc;
-    __shared__ uint32_t wl[NW][kListCap];
+    __shared__ uint32_t wl[NW][kListCap + (BS_P1_TRASH ? 64 : 0)];   // + the trash slots of the exec-free stores
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
@@ -314,8 +353,14 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         r.valid = idx < lim;
         r.e = list[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
+#if BS_LIST_POS
+        const uint32_t pos = r.e >> 6;
+        const uint32_t e = tco[pos].y >> 22;        // tile entry (cell) of the candidate
+        r.j = sbase[e] + base + pos;
+#else
         const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
         r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
+#endif
         r.a = rec[3 * (size_t)r.j];
         r.b = rec[3 * (size_t)r.j + 1];
         r.c = rec[3 * (size_t)r.j + 2];
@@ -381,7 +426,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         __syncthreads();
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
-        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+        const uint32_t tail = filter_window<kListDrain, kListCap>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
                                                         [&](uint32_t t) __attribute__((always_inline)) {
             // phase 2 on every full step of rounds; the rest moves to the front
             const uint32_t lim = t - t % kStep, rem = t - lim;
@@ -462,7 +507,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     } sh;
     auto& tco = sh.tco;
     auto& acc = sh.acc;
-    __shared__ uint32_t wl[NW][kListCap];
+    __shared__ uint32_t wl[NW][kListCap + (BS_P1_TRASH ? 64 : 0)];   // + the trash slots of the exec-free stores
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
     __shared__ uint32_t wsum[17];
@@ -580,8 +625,14 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         r.valid = idx < lim;
         r.e = lst[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
+#if BS_LIST_POS
+        const uint32_t pos = r.e >> 6;
+        const uint32_t e = tco[pos].y >> 22;        // tile entry (cell) of the candidate
+        r.j = sbase[e] + base + pos;
+#else
         const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
         r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
+#endif
         r.q = lat.p[r.j];
         r.w = lat.v[r.j];
         return r;
@@ -590,7 +641,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     auto eval = [&](Rec r) {
         // every lane of the wave is active here: each reads its owner's registers
         const uint32_t ol = r.e & 63u;
+#if BS_P2_SEPW
+        // the sub-lattice residuals (.w) only feed separation terms: read below
+        r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), 0u);
+#else
         r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), lane_read(me.w, ol));
+#endif
         r.ov = make_float4(lane_read(fv4.x, ol), lane_read(fv4.y, ol), lane_read(fv4.z, ol), lane_read(fv4.w, ol));
         const int32_t ix = (int32_t)(r.q.x - r.m.x), iy = (int32_t)(r.q.y - r.m.y), iz = (int32_t)(r.q.z - r.m.z);
         const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
@@ -601,9 +657,24 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const float c_dn = c - band, c_up = c + band;
         const bool skip = !r.valid | (r.j == t0 + r.o);    // padding, or the owner itself
         bool in_c, out_c, in_s, out_s, in_a, out_a;
-        rule_bands(P, COH, s2, c, c_dn, c_up, skip, in_c, out_c);
-        rule_bands(P, SEP, s2, c, c_dn, c_up, skip, in_s, out_s);
-        rule_bands(P, ALI, s2, c, c_dn, c_up, skip, in_a, out_a);
+#if BS_P2_CONE1
+        if (P.same_cone) {
+            // one view cone for all three rules: its compares are shared
+            const bool cin = (c_dn >= P.c_hi[0]) & (c < P.c_top[0]);
+            const bool cout = c_up <= P.c_lo[0];
+            in_c = (s2 < P.s_lo[COH]) & cin & !skip;
+            out_c = !(s2 < P.s_hi[COH]) | cout | skip;
+            in_s = (s2 < P.s_lo[SEP]) & cin & !skip;
+            out_s = !(s2 < P.s_hi[SEP]) | cout | skip;
+            in_a = (s2 < P.s_lo[ALI]) & cin & !skip;
+            out_a = !(s2 < P.s_hi[ALI]) | cout | skip;
+        } else
+#endif
+        {
+            rule_bands(P, COH, s2, c, c_dn, c_up, skip, in_c, out_c);
+            rule_bands(P, SEP, s2, c, c_dn, c_up, skip, in_s, out_s);
+            rule_bands(P, ALI, s2, c, c_dn, c_up, skip, in_a, out_a);
+        }
         const bool obad = !(r.ov.w > 0.f);
         const bool sure = (in_c | out_c) & (in_s | out_s) & (in_a | out_a) & !(obad & !(out_c & out_s & out_a));
         if (!sure) {
@@ -613,6 +684,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
                          in.z[r.j], in_s, in_a, in_c);
             ++nre;
         }
+#if BS_P2_SEPW
+        if (wave_mask(in_s) != 0) r.m.w = lane_read(me.w, ol);   // wave-uniform: every lane active
+#endif
         accumulate(r.o, r.m, r.q, r.w, r.j, in_c, in_s, in_a);
     };
     // Evaluates list[0, lim) in rounds of 64 lanes, kRounds rounds per
@@ -647,7 +721,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
         // phase 1 (filter_window), phase 2 whenever the list is full
-        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+        const uint32_t tail = filter_window<kListDrain, kListCap>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
                                                         [&](uint32_t t) __attribute__((always_inline)) {
             // phase 2 on every full step of rounds; the rest moves to the front
             const uint32_t lim = t - t % kStep, rem = t - lim;
