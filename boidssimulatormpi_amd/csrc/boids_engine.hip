@@ -45,12 +45,43 @@ static_assert(BS_MIXED_CHUNK_SPAN >= 1 && BS_MIXED_CHUNK_SPAN <= boids::kChunkSp
 #include <algorithm>
 #include <cstdint>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace boids;
 
 static thread_local std::string g_create_error;
+
+// Static device scratch of the allocation-free entry point
+// (bs_metrics_flock_noalloc): kScratchBytes on every GPU the library runs
+// on, held until the process ends.  On the host, `scratch` names a host-side
+// stand-in, not a device address: its address on each device is looked up
+// once (hipGetSymbolAddress, with that device current) and only that address
+// is handed to kernels.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_addr;   // device -> address of `scratch` there
+
+// Address of `scratch` on `device`, which must be the current device;
+// nullptr (and *why set) if the lookup fails.
+static void* scratch_address(int device, std::string* why)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t st = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (st != hipSuccess) {
+        *why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(st);
+        return nullptr;
+    }
+    g_scratch_addr[device] = ptr;
+    return ptr;
+}
 
 struct bs_engine {
     bs_params prm{};
@@ -100,6 +131,8 @@ struct bs_engine {
     float4 *rec4 = nullptr, *vrec4 = nullptr;   // float grid: sorted positions (+ id) and velocities
     uint32_t* recheck = nullptr;
     uint32_t* overflow = nullptr;      // chunk-table count beyond its capacity (0: none)
+    void* scratch = nullptr;           // this device's static scratch (allocation-free entry points)
+    std::string scratch_why;           // why it is unavailable
     // outputs
     void* forces = nullptr;   // float or double, 9n, id order
     uint64_t* digest = nullptr;
@@ -690,6 +723,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
     e->device = p->device;
     st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (st != hipSuccess) { std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(st); e->stream = nullptr; release(e); g_create_error = m; return BS_E_HIP; }
+    e->scratch = scratch_address(p->device, &e->scratch_why);   // once per device, this device current
 
     const uint32_t n = e->prm.n;
     auto bail = [&](int code) { std::string m = e->err; release(e); g_create_error = m; return code; };
@@ -1255,10 +1289,12 @@ extern "C" int bs_debug_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
     return rc;
 }
 
-// Flock metrics of the current state (see boids_metrics.hip).  Temporary
-// device memory is allocated and released inside the call.  Flock k of a
+// Flock metrics of the current state (see boids_metrics.hip).  The
+// workspace is temporary device memory allocated and released inside the
+// call, or (static_ws) the device's static scratch array.  Flock k of a
 // batch is the id range [boid_base, boid_base + n) of the unsorted state.
-extern "C" int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out)
+static int metrics_impl(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out,
+                        bool static_ws)
 {
     if (!e || !out) return BS_E_ARG;
     if (!e->has_state) return fail(e, BS_E_STATE, "no state: call bs_set_state_* first");
@@ -1276,26 +1312,46 @@ extern "C" int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32
     const double cap = std::max(1.0, 2.0 * floor(cbrt((double)n)));
     g = std::min(std::min(g, cap), 1024.0);
     const int G = std::max(1, (int)g);
+    const size_t bytes = metrics_workspace_bytes(n, G);
+    if (static_ws && bytes > kScratchBytes)
+        return fail(e, BS_E_ARG, "metrics workspace of " + std::to_string(bytes) + " bytes exceeds the static scratch (" +
+                                     std::to_string(kScratchBytes) + " bytes): use bs_metrics_flock");
+    if (static_ws && !e->scratch) return fail(e, BS_E_STATE, "static scratch unavailable: " + e->scratch_why);
     if (is_f32(e->mode))
         HIP_TRY(e, launch_f32_to_f64(ntot, e->fa, e->tmp_pos, e->tmp_vel, e->stream));
     else
         HIP_TRY(e, launch_unsort_f64(ntot, e->da, e->tmp_pos, e->tmp_vel, e->stream));
     void* ws = nullptr;
-    const size_t bytes = metrics_workspace_bytes(n, G);
-    if (hipMalloc(&ws, bytes) != hipSuccess) return fail(e, BS_E_NOMEM, "metrics workspace allocation failed");
+    if (static_ws) ws = e->scratch;
+    else if (hipMalloc(&ws, bytes) != hipSuccess) return fail(e, BS_E_NOMEM, "metrics workspace allocation failed");
     double* res = nullptr;
     hipError_t st = launch_metrics(n, F, G, eps, min_size, e->tmp_pos + 3 * (size_t)base,
-                                   e->tmp_vel + 3 * (size_t)base, ws, bytes, &res, e->stream);
+                                   e->tmp_vel + 3 * (size_t)base, ws, static_ws ? kScratchBytes : bytes, &res,
+                                   e->stream);
     double host[4] = {0, 0, 0, 0};
     if (st == hipSuccess) st = hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, e->stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
-    (void)hipFree(ws);
+    if (!static_ws) (void)hipFree(ws);
     if (st != hipSuccess) return hip_fail(e, st, "metrics");
     out->polarisation = host[0];
     out->mean_nn = host[1];
     out->clusters = (uint64_t)host[2];
     out->largest = (uint64_t)host[3];
     return BS_OK;
+}
+
+extern "C" int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out)
+{
+    return metrics_impl(e, flock, eps, min_size, out, false);
+}
+
+// The same without any device allocation: the workspace is this device's
+// static scratch array (its address looked up once per device at engine
+// creation).  Calls that overlap on one GPU share the array and race.
+extern "C" int bs_metrics_flock_noalloc(bs_engine* e, uint32_t flock, double eps, uint32_t min_size,
+                                        bs_metrics_out* out)
+{
+    return metrics_impl(e, flock, eps, min_size, out, true);
 }
 
 extern "C" int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out)

@@ -82,7 +82,8 @@ EXPORTS = ["bs_create", "bs_destroy", "bs_last_error", "bs_generate_init", "bs_r
            "bs_get_state_f64", "bs_get_forces_f32", "bs_get_forces_f64", "bs_get_digest",
            "bs_get_info", "bs_get_timing", "bs_reset_timing", "bs_debug_thresholds",
            "bs_debug_rechecks", "bs_debug_copy", "bs_metrics", "bs_version", "bs_create_batch",
-           "bs_get_flock_count", "bs_get_flock_info", "bs_metrics_flock", "bs_debug_grid"]
+           "bs_get_flock_count", "bs_get_flock_info", "bs_metrics_flock", "bs_debug_grid",
+           "bs_metrics_flock_noalloc"]
 
 
 class GridInfo(ctypes.Structure):
@@ -138,6 +139,7 @@ def lib():
         "bs_get_flock_count": (c_int, [vp, P(ctypes.c_uint32)]),
         "bs_get_flock_info": (c_int, [vp, ctypes.c_uint32, P(FlockInfo)]),
         "bs_metrics_flock": (c_int, [vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
+        "bs_metrics_flock_noalloc": (c_int, [vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, P(Metrics)]),
         "bs_debug_grid": (c_int, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32),
                                   P(ctypes.c_uint32), P(GridInfo)]),
     }
@@ -316,6 +318,14 @@ class Engine:
         self._check(lib().bs_debug_grid(self._h, _p(key, u32), _p(ids, u32), _p(start, u32), _p(table, u32),
                                         ctypes.byref(info)), "bs_debug_grid")
         return info, key, ids, start, table[:min(info.nchunks, info.max_chunks)].copy()
+
+    def metrics_noalloc(self, eps, min_size=2, flock=0) -> dict:
+        """metrics() of flock `flock` with no device allocation (the library's
+        static scratch array; calls overlapping on one GPU race)."""
+        m = Metrics()
+        self._check(lib().bs_metrics_flock_noalloc(self._h, flock, eps, min_size, ctypes.byref(m)),
+                    "bs_metrics_flock_noalloc")
+        return m.as_dict()
 
     def flock_count(self) -> int:
         c = ctypes.c_uint32()

@@ -241,6 +241,16 @@ typedef struct {
 int bs_metrics(bs_engine* e, double eps, uint32_t min_size, bs_metrics_out* out);
 /* The same for flock k of a batch (bs_metrics fails on a batch of more than one flock). */
 int bs_metrics_flock(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out);
+/* bs_metrics_flock without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (analysis inside tight
+ * stepping loops, harnesses that forbid hipMalloc).  Its workspace is a
+ * static device array of 64 MiB on every GPU the library runs on (held
+ * until the process ends) instead of temporary memory; a flock whose
+ * workspace exceeds it is rejected with BS_E_ARG (the workspace takes at
+ * most ~128 bytes per boid: flocks of up to ~5 x 10^5 boids always fit).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Results equal bs_metrics_flock's. */
+int bs_metrics_flock_noalloc(bs_engine* e, uint32_t flock, double eps, uint32_t min_size, bs_metrics_out* out);
 
 /* Measurement helper: `reps` launches of a plain streaming copy of `bytes`
  * bytes on the current device (reference point for HBM byte counters).
