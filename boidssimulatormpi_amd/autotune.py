@@ -28,7 +28,7 @@ CACHE_PATH = os.environ.get("BOIDS_TUNING_CACHE") or os.path.join(
     os.path.expanduser("~"), ".cache", "boidssimulatormpi_amd", "tuning.json")
 
 BLOCKS = (64, 128, 256)
-TILES = (6, 9, 12)          # LDS candidates staged per thread and pass
+TILES = (5, 6, 9, 12)       # LDS candidates staged per thread and pass
 FILLS = (60, 80, 100)       # chunk fill target, percent of the workgroup
 CELL_SCALES = (1.0, 0.85)   # cells per axis relative to the automatic choice
 

@@ -5,12 +5,12 @@
 #include "boids_internal.h"
 
 // Automatic LDS tile depth of the neighbour kernels (candidates per thread
-// and pass): 6, 9 or 12.
+// and pass): 5, 6, 9 or 12.
 #ifndef BS_MIXED_TILE
-#define BS_MIXED_TILE 6
+#define BS_MIXED_TILE 5
 #endif
 #ifndef BS_F64_TILE
-#define BS_F64_TILE 6
+#define BS_F64_TILE 5
 #endif
 // x columns per cell edge of the grid modes (DevParams::sx): both paths bin
 // x at half the cell edge, a 2.5-edge window along x instead of 3 (with
@@ -624,7 +624,7 @@ static int create_engine(const bs_params* fp, uint32_t count, bool batch, bs_eng
             release(e);
             return fail(nullptr, BS_E_ARG, "block must be 0, 64, 128 or 256");
         }
-        if (p->tile != 0 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 6, 9 or 12"); }
+        if (p->tile != 0 && p->tile != 5 && p->tile != 6 && p->tile != 9 && p->tile != 12) { release(e); return fail(nullptr, BS_E_ARG, "tile must be 0, 5, 6, 9 or 12"); }
         if (p->fill != 0 && (p->fill < 25 || p->fill > 100)) { release(e); return fail(nullptr, BS_E_ARG, "fill must be 0 or 25..100"); }
         e->tile = p->tile ? p->tile : (p->mode == BS_MODE_GRID_MIXED ? BS_MIXED_TILE : BS_F64_TILE);
         // Automatic fill: full chunks, unless that leaves fewer than ~2048

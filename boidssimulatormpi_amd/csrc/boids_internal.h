@@ -123,7 +123,9 @@ hipError_t launch_rows(Flocks F, const uint32_t* start, uint32_t* row_count, uin
                        int chunk, int cspan, int max_cols, hipStream_t stream);
 hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start, uint32_t* count,
                           const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
-// Scatter by the arrival ranks the counting pass recorded (no atomics).
+// Scatter by the arrival ranks the counting pass recorded (no atomics),
+// then sort every run of 64 slots by (cell, id) (the gather merges a cell's
+// sorted parts into its in-cell id order).
 hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
                                  const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
 // lat (mixed) receives the lattice copy; rec (FP64 grid) the interleaved
@@ -132,7 +134,7 @@ hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid,
                          const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
                          uint32_t* key_sorted, Lattice* lat, double2* rec, hipStream_t stream);
 // Neighbour kernels (boids_nbr.hip).  nb = workgroup size (64, 128 or 256),
-// tile = LDS candidates per thread and pass (6, 9 or 12), max_cols = most x
+// tile = LDS candidates per thread and pass (5, 6, 9 or 12), max_cols = most x
 // columns of a chunk's tile (columns a chunk covers + 2 sx), max_chunks =
 // capacity of `table`.
 hipError_t launch_nbr_f64(Flocks F, int nb, int tile, int max_cols, StateF64 in, const double2* rec,

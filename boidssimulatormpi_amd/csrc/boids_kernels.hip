@@ -263,9 +263,81 @@ __global__ __launch_bounds__(kBlock) void k_scatter_ranked(uint32_t n, const uin
     permid[t] = id[s];
 }
 
+// In-cell order, step 1: every run of 64 consecutive slots (one wave) is
+// sorted by (cell, id) in registers, in place: a bitonic network across the
+// wave's lanes over (cell part within the run, id) -- the slots are grouped
+// by cell already, so this sorts each cell's part inside the run.  A cell
+// spanning runs leaves one ascending part per run, which k_gather merges.
+// Keys are 32-bit (6-bit part index, 26-bit id) when every id is below 2^26,
+// else 64-bit.  A run already in order (the common case: boids arriving from
+// one old cell are ranked in lane order) is left alone.
+template <typename K>
+__device__ __forceinline__ K shfl_xor_key(K v, uint32_t m)
+{
+    if constexpr (sizeof(K) == 4) {
+        return (K)__shfl_xor((uint32_t)v, m, 64);
+    } else {
+        const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+        return ((K)hi << 32) | lo;
+    }
+}
 
-// Final slot = cell start + rank of the boid id among its cell's ids, so
-// the order inside a cell is ascending id whatever the atomics did.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_sort_runs(uint32_t n, const uint32_t* __restrict__ key,
+                                                      uint32_t* perm, uint32_t* permid)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool in = t < n;
+    uint32_t p = 0u, id = 0u, c = 0xFFFFFFFFu;
+    if (in) {
+        p = perm[t];
+        id = permid[t];
+        c = key[p];
+    }
+    const uint32_t cp = __shfl_up(c, 1, 64), ip = __shfl_up(id, 1, 64);
+    const bool fresh = lane == 0 || cp != c;         // first slot of a cell's part
+    if (__builtin_amdgcn_ballot_w64(in && !fresh && ip > id) == 0) return;   // wave-uniform
+    const uint64_t starts = __builtin_amdgcn_ballot_w64(fresh);
+    const uint32_t part = __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+                          (fresh ? 1u : 0u) - 1u;     // parts begun at or before this lane, minus one
+    constexpr uint32_t kShift = sizeof(K) == 4 ? 26u : 32u;
+    K k = in ? (((K)part << kShift) | (K)id) : ~(K)0;   // past the end: sorts last, never written
+    for (uint32_t size = 2; size <= 64; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const K pk = shfl_xor_key(k, stride);
+            const uint32_t pp = __shfl_xor(p, stride, 64);
+            // ascending blocks of `size` where lane & size == 0: the lower
+            // lane of a pair keeps the smaller key there, the larger elsewhere
+            const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+            if (keep_min ? pk < k : pk > k) { k = pk; p = pp; }
+        }
+    }
+    if (in) {
+        perm[t] = p;
+        permid[t] = (uint32_t)(k & (((K)1 << kShift) - 1));
+    }
+}
+
+// Ids below `me` in the ascending run ids[0, len).
+__device__ __forceinline__ uint32_t count_below(const uint32_t* ids, uint32_t len, uint32_t me)
+{
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ids[mid] < me) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// In-cell order, step 2: final slot = cell start + rank of the boid id among
+// its cell's ids, so the order inside a cell is ascending id whatever the
+// atomics did.  The rank is the slot's position in its own sorted part plus
+// a binary search in each other part of the cell (k_sort_runs): O(1) for a
+// cell inside one run, O(occupancy / 64 * log 64) for a dense one, instead
+// of a pass over the whole cell per boid (a scan of short parts instead of
+// the search measured slower: its loop is as long as the part).
 __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* perm,
                                                    const uint32_t* permid, const uint32_t* key,
                                                    const uint32_t* start, uint32_t* count,
@@ -279,8 +351,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* per
     const uint32_t me = permid[t];
     const uint32_t c = key[s];
     const uint32_t lo = start[c], hi = start[c + 1];
-    uint32_t rank = 0;
-    for (uint32_t u = lo; u < hi; ++u) rank += permid[u] < me ? 1u : 0u;
+    const uint32_t run = t >> 6;
+    uint32_t rank = t - max(lo, run << 6);
+    for (uint32_t v = lo >> 6; (v << 6) < hi; ++v) {
+        if (v == run) continue;
+        const uint32_t a = max(lo, v << 6), b = min(hi, (v + 1) << 6);
+        rank += count_below(permid + a, b - a, me);
+    }
     const uint32_t f = lo + rank;
     const double x = src.x[s], y = src.y[s], z = src.z[s];
     const double vx = src.vx[s], vy = src.vy[s], vz = src.vz[s];
@@ -392,6 +469,12 @@ hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter_ranked, dim3(blocks_for(n)), dim3(kBlock), 0, stream, n, key, rank, start, id,
                        perm, permid);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    if (n <= (1u << 26))
+        hipLaunchKernelGGL(k_sort_runs<uint32_t>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, n, key, perm, permid);
+    else
+        hipLaunchKernelGGL(k_sort_runs<uint64_t>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, n, key, perm, permid);
     return hipGetLastError();
 }
 

@@ -27,6 +27,8 @@
 // Tiles larger than the LDS capacity are processed in passes.
 #include "boids_device.h"
 
+#include <type_traits>
+
 namespace boids {
 
 template <int NB, int TD> struct NbrCfg {
@@ -90,16 +92,11 @@ __device__ __forceinline__ uint2 coarse_of(uint32_t lx, uint32_t ly, uint32_t lz
 // packed; the dot product saturates instead of wrapping).  z is the low
 // half of q.y (the high half holds the tile entry): its difference is taken
 // mod 2^16 and squared as a sign-extended 16-bit value, dz^2 <= 2^30.
-__device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, v2s my_z)
+__device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, uint32_t my_z)
 {
     const v2s dxy = __builtin_bit_cast(v2s, q.x) - my_xy;
-#if BS_P1_Z16
-    const int32_t dz = (int16_t)(uint16_t)(q.y - (uint32_t)(uint16_t)my_z.x);
+    const int32_t dz = (int16_t)(uint16_t)(q.y - my_z);   // v_sub + v_mul_i32_i24 on the sign-extended words
     return __builtin_amdgcn_sdot2(dxy, dxy, dz * dz, true);
-#else
-    const v2s dzz = __builtin_bit_cast(v2s, q.y & 0xFFFFu) - my_z;
-    return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dzz, dzz, 0, true), true);
-#endif
 }
 
 // Stages tile positions [base, base + here); each thread copies one run of
@@ -155,11 +152,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Per-wave hit list of the neighbour kernels, entries (tile entry << 22) |
-// (tile index << 6) | owner lane, appended by ballot (each hit stored under
-// its own lane mask) and drained in full rounds of 64 (kRounds rounds per
-// step) once it holds kDrain entries.  A filter step appends at most 4 x 64
-// entries to a list holding fewer than kDrain, so kCap bounds every index.
+// Per-wave hit list of the neighbour kernels, entries (tile position << 6)
+// | owner lane, appended by ballot (each hit stored under its own lane
+// mask) and drained in full rounds of 64 (kRounds rounds per step) once it
+// holds kDrain entries.  A filter step appends at most 4 x 64 entries to a
+// list holding fewer than kDrain, so kCap bounds every index.  The entries
+// are 16-bit where the tile positions fit 10 bits (NB x TD <= 1024, the
+// default shapes): half the list's LDS, one more workgroup per CU.
 #ifndef BS_LIST_DRAIN
 #define BS_LIST_DRAIN 128
 #endif
@@ -176,45 +175,23 @@ template <uint32_t R> struct ListCfg {
     static constexpr uint32_t kCap = kDrain + 4 * 64;     // + one filter step (4 candidates per lane)
     static_assert(kDrain % kStep == 0, "list drain threshold");
 };
+template <int NB, int TD>
+using ListEntry = typename std::conditional<(NB * TD <= 1024), uint16_t, uint32_t>::type;
 
 // Phase 1 of the neighbour kernels over one lane's window [k, k1) of the
 // staged tile (positions relative to the pass's first): four candidates per
 // lane and step, the next four read ahead (positions stay <= k1 + 3, inside
 // the tile's 4 entries of padding); the list is flushed whenever it holds
 // kDrain entries, so a step always starts below kDrain and appends at most
-// 4 x 64 more.  A candidate within the coarse bound is appended as
-// (tile entry << 22) | (tile position << 6) | lane, in (step, candidate,
-// lane) order; the owner itself is a hit (coarse distance 0) and is skipped
-// by the evaluation.  flush(tail) evaluates the list's full rounds and
-// returns the number of entries it leaves at the front.  Returns the final
-// tail.  Every lane of the wave takes part.
-#ifndef BS_P1_Z16
-#define BS_P1_Z16 0
-#endif
-#ifndef BS_P1_TRASH
-#define BS_P1_TRASH 0
-#endif
-#ifndef BS_LIST_POS
-#define BS_LIST_POS 0
-#endif
-#ifndef BS_P2_SEPW
-#define BS_P2_SEPW 0
-#endif
-#ifndef BS_P2_CONE1
-#define BS_P2_CONE1 0
-#endif
-// List entry of a hit at tile position k (e = k << 6 | lane): with
-// BS_LIST_POS the entry is e alone and phase 2 reads the tile entry (cell)
-// from the tile; otherwise the tile entry rides in the top 10 bits.
-#if BS_LIST_POS
-#define BS_LIST_VAL(a, e) (e)
-#else
-#define BS_LIST_VAL(a, e) (((a).y & 0xFFC00000u) | (e))
-#endif
-
-template <uint32_t kDrain, uint32_t kTrash, typename FLUSH>
-__device__ __forceinline__ uint32_t filter_window(const uint2* tco, uint32_t* list, uint32_t k, uint32_t k1,
-                                                  uint32_t lane, v2s my_xy, v2s my_z, int32_t thr, FLUSH&& flush)
+// 4 x 64 more.  A candidate within the coarse bound is appended as (tile
+// position << 6) | lane, in (step, candidate, lane) order (phase 2 reads the
+// candidate's tile entry from the tile); the owner itself is a hit (coarse
+// distance 0) and is skipped by the evaluation.  flush(tail) evaluates the
+// list's full rounds and returns the number of entries it leaves at the
+// front.  Returns the final tail.  Every lane of the wave takes part.
+template <uint32_t kDrain, typename LE, typename FLUSH>
+__device__ __forceinline__ uint32_t filter_window(const uint2* tco, LE* list, uint32_t k, uint32_t k1,
+                                                  uint32_t lane, v2s my_xy, uint32_t my_z, int32_t thr, FLUSH&& flush)
 {
     auto near = [&](uint2 q) { return coarse_d2(q, my_xy, my_z) <= thr; };
     uint32_t tail = 0;
@@ -233,19 +210,11 @@ __device__ __forceinline__ uint32_t filter_window(const uint2* tco, uint32_t* li
         const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
         const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
         const uint32_t e = (k << 6) | lane;
-#if BS_P1_TRASH
-        // no exec-mask changes: a miss is stored to its lane's slot past the list
-        list[h0 ? tail + lanes_below(m0) : kTrash + lane] = BS_LIST_VAL(a0, e);
-        list[h1 ? t1 + lanes_below(m1) : kTrash + lane] = BS_LIST_VAL(a1, e + (1u << 6));
-        list[h2 ? t2 + lanes_below(m2) : kTrash + lane] = BS_LIST_VAL(a2, e + (2u << 6));
-        list[h3 ? t3 + lanes_below(m3) : kTrash + lane] = BS_LIST_VAL(a3, e + (3u << 6));
-#else
         // the wave-uniform list offsets go into the (scalar) base address
-        if (h0) (list + tail)[lanes_below(m0)] = BS_LIST_VAL(a0, e);
-        if (h1) (list + t1)[lanes_below(m1)] = BS_LIST_VAL(a1, e + (1u << 6));
-        if (h2) (list + t2)[lanes_below(m2)] = BS_LIST_VAL(a2, e + (2u << 6));
-        if (h3) (list + t3)[lanes_below(m3)] = BS_LIST_VAL(a3, e + (3u << 6));
-#endif
+        if (h0) (list + tail)[lanes_below(m0)] = (LE)e;
+        if (h1) (list + t1)[lanes_below(m1)] = (LE)(e + (1u << 6));
+        if (h2) (list + t2)[lanes_below(m2)] = (LE)(e + (2u << 6));
+        if (h3) (list + t3)[lanes_below(m3)] = (LE)(e + (3u << 6));
         tail = t3 + __popcll(m3);
         if (tail >= kDrain) tail = flush(tail);
         k = kn;
@@ -277,21 +246,24 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     using LC = ListCfg<BS_F64_ROUNDS>;
     constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
     constexpr int NW = NB / 64;
+    using LE = ListEntry<NB, TD>;
     // The tile first, at LDS address 0 (the largest variable is placed
     // first), so phase 1's reads need no base-address additions.  tco:
     // coarse positions, tile entry in bits 22-31 of .y; + 4: a finished
-    // lane's look-ahead reads (positions stay <= base + here).
+    // lane's look-ahead reads (positions stay <= base + here).  Before the
+    // first pass stages it, the tile's first words serve as the tile-table
+    // scan's scratch.
     __shared__ struct {
         uint2 tco[Cfg::kCap + 4];
         double acc[9][NB];                      // sums of q (cohesion), e/|e| (separation), v (alignment)
     } sh;
     auto& tco = sh.tco;
     auto& acc = sh.acc;
-    __shared__ uint32_t wl[NW][kListCap + (BS_P1_TRASH ? 64 : 0)];   // + the trash slots of the exec-free stores
+    __shared__ LE wl[NW][kListCap];
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
-    __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
+    static_assert(sizeof(sh.tco) >= 17 * sizeof(uint32_t), "scan scratch inside the tile");
 
     const uint32_t nch = *nchunks;
     const uint32_t b = xcd_chunk(blockIdx.x, nch);
@@ -331,7 +303,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const double sc = P.lat_scale;
     const uint2 mc = coarse_of(to_lattice(px, sc), to_lattice(py, sc), to_lattice(pz, sc));
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
-    const v2s my_z = __builtin_bit_cast(v2s, mc.y);
+    const uint32_t my_z = mc.y;
     const double vn = norm3(vx, vy, vz);            // owner data stays in registers (read by ds_bpermute)
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
@@ -340,27 +312,26 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     const double F = P.F, half = P.half;
     const bool k_c = P.cone[COH], k_s = P.cone[SEP], k_a = P.cone[ALI];
 
-    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, (uint32_t*)tco);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
-    uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
+    LE* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
 
     // One (owner, candidate) pair per lane: the candidate's record from L2
     // by slot, read ahead of the evaluation.
     struct Rec { double2 a, b, c; uint32_t e, o, j; bool valid; };
+    // sorted slot of a list entry's candidate (pass base `base`): the tile
+    // entry (cell) of its tile position gives the cell's slot base
+    auto slot_of = [&](uint32_t ent, uint32_t base) {
+        const uint32_t pos = ent >> 6;
+        return sbase[tco[pos].y >> 22] + base + pos;
+    };
     auto load = [&](uint32_t idx, uint32_t lim, uint32_t base) {
         Rec r;
         r.valid = idx < lim;
         r.e = list[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
-#if BS_LIST_POS
-        const uint32_t pos = r.e >> 6;
-        const uint32_t e = tco[pos].y >> 22;        // tile entry (cell) of the candidate
-        r.j = sbase[e] + base + pos;
-#else
-        const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
-        r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
-#endif
+        r.j = slot_of(r.e, base);
         r.a = rec[3 * (size_t)r.j];
         r.b = rec[3 * (size_t)r.j + 1];
         r.c = rec[3 * (size_t)r.j + 2];
@@ -426,12 +397,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         __syncthreads();
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
-        const uint32_t tail = filter_window<kListDrain, kListCap>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
                                                         [&](uint32_t t) __attribute__((always_inline)) {
             // phase 2 on every full step of rounds; the rest moves to the front
             const uint32_t lim = t - t % kStep, rem = t - lim;
             drain(lim, base);
-            uint32_t mv[kRounds];
+            LE mv[kRounds];
 #pragma unroll
             for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= t < kListCap
 #pragma unroll
@@ -497,21 +468,23 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     using LC = ListCfg<BS_MIXED_ROUNDS>;
     constexpr uint32_t kRounds = LC::kRounds, kStep = LC::kStep, kListDrain = LC::kDrain, kListCap = LC::kCap;
     constexpr int NW = NB / 64;
+    using LE = ListEntry<NB, TD>;
     // The tile first, at LDS address 0 (the largest variable is placed
     // first), so phase 1's reads need no base-address additions.  tco:
     // coarse positions, tile entry e in bits 22-31 of .y (z in bits 0-15);
-    // + 4: a finished lane's look-ahead reads.
+    // + 4: a finished lane's look-ahead reads.  Before the first pass stages
+    // it, the tile's first words serve as the tile-table scan's scratch.
     __shared__ struct {
         uint2 tco[Cfg::kCap + 4];
         double acc[9][NB];                      // integer-valued sums: cohesion, separation, alignment
     } sh;
     auto& tco = sh.tco;
     auto& acc = sh.acc;
-    __shared__ uint32_t wl[NW][kListCap + (BS_P1_TRASH ? 64 : 0)];   // + the trash slots of the exec-free stores
+    __shared__ LE wl[NW][kListCap];
     __shared__ uint64_t acc_n[NB];              // nc | ns << 21 | na << 42
     __shared__ uint64_t acc_h[DIGEST ? 3 * NB : 1];
-    __shared__ uint32_t wsum[17];
     extern __shared__ __align__(16) unsigned char dyn[];
+    static_assert(sizeof(sh.tco) >= 17 * sizeof(uint32_t), "scan scratch inside the tile");
 
     const uint32_t nch = *nchunks;
     const uint32_t b = xcd_chunk(blockIdx.x, nch);
@@ -539,70 +512,66 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const uint32_t lane = tid & 63u, wbase = tid - lane;
     const uint32_t s = t0 + tid;
     const bool active = s < t1;
-    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t myid = 0;
     uint4 me = make_uint4(0, 0, 0, 0);
     int L = sx;
-    if (active) {
-        px = in.x[s]; py = in.y[s]; pz = in.z[s];
-        vx = in.vx[s]; vy = in.vy[s]; vz = in.vz[s];
-        myid = in.id[s];
-        me = lat.p[s];
-        L = (int)((key_sorted[s] - cbase) % Gx) - cxa + sx;
-    }
-    const uint2 mc = coarse_of(me.x, me.y, me.z);
-    const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
-    const v2s my_z = __builtin_bit_cast(v2s, mc.y);
     // Owner data that evaluating lanes read from this lane's registers: the
     // lattice record and the FP32 velocity with 1/|v|.  A vanishing FP32
     // velocity makes the FP32 cosine meaningless: 1/|v| = 0 marks it, and
-    // every pair of the owner is rechecked.
-    float4 fv4;
-    {
-        const float fvx = (float)vx, fvy = (float)vy, fvz = (float)vz;
+    // every pair of the owner is rechecked.  The FP64 state is read again
+    // for phase B instead of being held through both phases.
+    float4 fv4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) {
+        me = lat.p[s];
+        L = (int)((key_sorted[s] - cbase) % Gx) - cxa + sx;
+        const float fvx = (float)in.vx[s], fvy = (float)in.vy[s], fvz = (float)in.vz[s];
         const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
         fv4 = make_float4(fvx, fvy, fvz, (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : 0.f);
     }
+    const uint2 mc = coarse_of(me.x, me.y, me.z);
+    const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
+    const uint32_t my_z = mc.y;
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
     acc_n[tid] = 0;
     if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     uint32_t nre = 0;
 
-    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, wsum);
+    build_tile_tables<NB>(start, cbase, G, sx, cxa, nseg, cy, cz, off, sbase, (uint32_t*)tco);
     const uint32_t total = off[E];
     const uint32_t wlo = off[9 * (L - sx)], whi = off[9 * (L + sx + 1)];
-    uint32_t* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
+    LE* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
     constexpr double kSub = 1.0 / (double)(1 << kSubBits);
 
+    // The separation term of a pair: the unit vector e/|e| in quanta of
+    // 2^-30, in FP64 (an FP32 unit vector carries ~1e-7 per term, and dense
+    // clusters sum tens of them), on the displacement refined by the
+    // sub-lattice residuals (a close pair's direction would otherwise carry
+    // the 1/2-unit rounding of both ends).
+    auto sep_term = [&](uint32_t o, const uint4& m, const uint4& q) {
+        const int32_t ix = (int32_t)(q.x - m.x), iy = (int32_t)(q.y - m.y), iz = (int32_t)(q.z - m.z);
+        const double ex = fma((double)(sub_field<0>(q.w) - sub_field<0>(m.w)), kSub, (double)ix);
+        const double ey = fma((double)(sub_field<1>(q.w) - sub_field<1>(m.w)), kSub, (double)iy);
+        const double ez = fma((double)(sub_field<2>(q.w) - sub_field<2>(m.w)), kSub, (double)iz);
+        const double e2 = fma(ez, ez, fma(ey, ey, ex * ex));
+        double y = __builtin_amdgcn_rsq(e2);           // estimate, then one Newton step
+        y = y * fma(-0.5 * e2, y * y, 1.5);
+        const double t = -y * kSepQ;
+        atomicAdd(&acc[3][o], rint(ex * t));
+        atomicAdd(&acc[4][o], rint(ey * t));
+        atomicAdd(&acc[5][o], rint(ez * t));
+    };
     // Adds the pair (owner o at lattice m, candidate record q, w, slot j)
     // to the owner's sums.  Every term is an integer-valued double, so the
     // LDS sums are exact and independent of the order pairs arrive in:
     // cohesion adds lattice displacements, alignment quantised velocities
-    // (lattice record), separation the unit vector e/|e| in quanta of
-    // 2^-30 (FP64), on the displacement refined by the sub-lattice residuals
-    // (a close pair's direction would otherwise carry the 1/2-unit rounding
-    // of both ends).
+    // (lattice record), separation the quantised unit vector.
     auto accumulate = [&](uint32_t o, const uint4& m, const uint4& q, const float4& w, uint32_t j, bool in_c,
                           bool in_s, bool in_a) {
         const int32_t ix = (int32_t)(q.x - m.x), iy = (int32_t)(q.y - m.y), iz = (int32_t)(q.z - m.z);
         if (in_c) {
             atomicAdd(&acc[0][o], (double)ix); atomicAdd(&acc[1][o], (double)iy); atomicAdd(&acc[2][o], (double)iz);
         }
-        if (in_s) {
-            // FP64: an FP32 unit vector carries ~1e-7 per term, and dense
-            // clusters sum tens of them
-            const double ex = fma((double)(sub_field<0>(q.w) - sub_field<0>(m.w)), kSub, (double)ix);
-            const double ey = fma((double)(sub_field<1>(q.w) - sub_field<1>(m.w)), kSub, (double)iy);
-            const double ez = fma((double)(sub_field<2>(q.w) - sub_field<2>(m.w)), kSub, (double)iz);
-            const double e2 = fma(ez, ez, fma(ey, ey, ex * ex));
-            double y = __builtin_amdgcn_rsq(e2);           // estimate, then one Newton step
-            y = y * fma(-0.5 * e2, y * y, 1.5);
-            const double t = -y * kSepQ;
-            atomicAdd(&acc[3][o], rint(ex * t));
-            atomicAdd(&acc[4][o], rint(ey * t));
-            atomicAdd(&acc[5][o], rint(ez * t));
-        }
+        if (in_s) sep_term(o, m, q);
         if (in_a) {
             atomicAdd(&acc[6][o], (double)w.x); atomicAdd(&acc[7][o], (double)w.y); atomicAdd(&acc[8][o], (double)w.z);
         }
@@ -620,19 +589,18 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     // candidate's lattice record from L2 by slot; all reads issued ahead
     // of the evaluation.
     struct Rec { uint4 q, m; float4 w, ov; uint32_t e, o, j; bool valid; };
-    auto load = [&](const uint32_t* lst, uint32_t idx, uint32_t lim, uint32_t base) {
+    // sorted slot of a list entry's candidate (pass base `base`): the tile
+    // entry (cell) of its tile position gives the cell's slot base
+    auto slot_of = [&](uint32_t ent, uint32_t base) {
+        const uint32_t pos = ent >> 6;
+        return sbase[tco[pos].y >> 22] + base + pos;
+    };
+    auto load = [&](const LE* lst, uint32_t idx, uint32_t lim, uint32_t base) {
         Rec r;
         r.valid = idx < lim;
         r.e = lst[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
-#if BS_LIST_POS
-        const uint32_t pos = r.e >> 6;
-        const uint32_t e = tco[pos].y >> 22;        // tile entry (cell) of the candidate
-        r.j = sbase[e] + base + pos;
-#else
-        const uint32_t e = r.e >> 22;               // tile entry (cell) of the candidate
-        r.j = sbase[e] + base + ((r.e >> 6) & 0xFFFFu);
-#endif
+        r.j = slot_of(r.e, base);
         r.q = lat.p[r.j];
         r.w = lat.v[r.j];
         return r;
@@ -641,12 +609,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     auto eval = [&](Rec r) {
         // every lane of the wave is active here: each reads its owner's registers
         const uint32_t ol = r.e & 63u;
-#if BS_P2_SEPW
-        // the sub-lattice residuals (.w) only feed separation terms: read below
-        r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), 0u);
-#else
         r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), lane_read(me.w, ol));
-#endif
         r.ov = make_float4(lane_read(fv4.x, ol), lane_read(fv4.y, ol), lane_read(fv4.z, ol), lane_read(fv4.w, ol));
         const int32_t ix = (int32_t)(r.q.x - r.m.x), iy = (int32_t)(r.q.y - r.m.y), iz = (int32_t)(r.q.z - r.m.z);
         const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
@@ -657,9 +620,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const float c_dn = c - band, c_up = c + band;
         const bool skip = !r.valid | (r.j == t0 + r.o);    // padding, or the owner itself
         bool in_c, out_c, in_s, out_s, in_a, out_a;
-#if BS_P2_CONE1
         if (P.same_cone) {
-            // one view cone for all three rules: its compares are shared
+            // one view cone for all three rules (a flock-uniform branch): its
+            // compares are shared
             const bool cin = (c_dn >= P.c_hi[0]) & (c < P.c_top[0]);
             const bool cout = c_up <= P.c_lo[0];
             in_c = (s2 < P.s_lo[COH]) & cin & !skip;
@@ -668,9 +631,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             out_s = !(s2 < P.s_hi[SEP]) | cout | skip;
             in_a = (s2 < P.s_lo[ALI]) & cin & !skip;
             out_a = !(s2 < P.s_hi[ALI]) | cout | skip;
-        } else
-#endif
-        {
+        } else {
             rule_bands(P, COH, s2, c, c_dn, c_up, skip, in_c, out_c);
             rule_bands(P, SEP, s2, c, c_dn, c_up, skip, in_s, out_s);
             rule_bands(P, ALI, s2, c, c_dn, c_up, skip, in_a, out_a);
@@ -684,9 +645,6 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
                          in.z[r.j], in_s, in_a, in_c);
             ++nre;
         }
-#if BS_P2_SEPW
-        if (wave_mask(in_s) != 0) r.m.w = lane_read(me.w, ol);   // wave-uniform: every lane active
-#endif
         accumulate(r.o, r.m, r.q, r.w, r.j, in_c, in_s, in_a);
     };
     // Evaluates list[0, lim) in rounds of 64 lanes, kRounds rounds per
@@ -721,12 +679,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
         // phase 1 (filter_window), phase 2 whenever the list is full
-        const uint32_t tail = filter_window<kListDrain, kListCap>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
+        const uint32_t tail = filter_window<kListDrain>(tco, list, kb - base, ke - base, lane, my_xy, my_z, P.coarse_thr,
                                                         [&](uint32_t t) __attribute__((always_inline)) {
             // phase 2 on every full step of rounds; the rest moves to the front
             const uint32_t lim = t - t % kStep, rem = t - lim;
             drain(lim, base);
-            uint32_t mv[kRounds];
+            LE mv[kRounds];
 #pragma unroll
             for (uint32_t i = 0; i < kRounds; ++i) mv[i] = list[lim + min(64 * i + lane, rem)];   // <= t < kListCap
 #pragma unroll
@@ -738,6 +696,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     }
     if (nre) atomicAdd(recheck, nre);
     if (!active) return;
+    const double px = in.x[s], py = in.y[s], pz = in.z[s];
+    const double vx = in.vx[s], vy = in.vy[s], vz = in.vz[s];
+    const uint32_t myid = in.id[s];
     // this wave's atomics are complete: LDS operations of a wave execute in order
     const uint64_t cnt = acc_n[tid];
     const uint32_t nc = (uint32_t)(cnt & 0x1FFFFFu), na = (uint32_t)(cnt >> 42);
@@ -797,12 +758,15 @@ static void launch_mixed_nb(Flocks F, size_t dyn, dim3 grid, StateF64 in, StateF
 // pairs; false for any other pair.
 #define BS_NBR_DISPATCH(nb, tile, CALL)                                        \
     do {                                                                      \
+        if (nb == 64 && tile == 5) { CALL(64, 5); break; }                    \
         if (nb == 64 && tile == 6) { CALL(64, 6); break; }                    \
         if (nb == 64 && tile == 9) { CALL(64, 9); break; }                    \
         if (nb == 64 && tile == 12) { CALL(64, 12); break; }                  \
+        if (nb == 128 && tile == 5) { CALL(128, 5); break; }                  \
         if (nb == 128 && tile == 6) { CALL(128, 6); break; }                  \
         if (nb == 128 && tile == 9) { CALL(128, 9); break; }                  \
         if (nb == 128 && tile == 12) { CALL(128, 12); break; }                \
+        if (nb == 256 && tile == 5) { CALL(256, 5); break; }                  \
         if (nb == 256 && tile == 6) { CALL(256, 6); break; }                  \
         if (nb == 256 && tile == 9) { CALL(256, 9); break; }                  \
         if (nb == 256 && tile == 12) { CALL(256, 12); break; }                \

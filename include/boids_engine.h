@@ -96,7 +96,7 @@ typedef struct {
     int32_t device;              /* HIP device ordinal */
     int32_t block;               /* grid modes: neighbour-kernel workgroup size, 64, 128 or 256; 0 = automatic (64) */
     int32_t tile;                /* grid modes: LDS tile depth, candidates staged per thread and pass:
-                                  * 6, 9 or 12; 0 = automatic (6) */
+                                  * 5, 6, 9 or 12; 0 = automatic (5) */
     int32_t fill;                /* grid modes: most boids per chunk, percent of the workgroup size
                                   * (25..100); 0 = automatic (100, less for small flocks: about 2048
                                   * chunks at least) */

@@ -107,3 +107,31 @@ def test_cells_per_axis_is_the_exact_cube_root_cap():
         e = B.Engine(p)
         assert e.info().cells_per_axis == T.auto_cells(p) == T.icbrt(n), n
         e.close()
+
+
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+def test_grid_build_crowded_cells_span_sort_windows(mode):
+    # cells of 3000, 1500, 700, 100, 65 and 64 boids (ids drawn at random):
+    # a cell of more than 64 boids takes its in-cell id order from 1024-slot
+    # sorted windows merged by binary search, a smaller one from a pass over
+    # its ids; a cell spanning three or four windows must still come out in
+    # ascending id order
+    n, F = 1 << 16, 1.0
+    rng = np.random.default_rng(11)
+    st = B.generate_init(E.INIT_RANDOM_UNIFORM, n, F, CFG0["vmax"], CFG0["vmin"], seed=5)
+    e = B.Engine(B.make_params(n, F, **CFG0, mode=mode))
+    G = e.info().cells_per_axis
+    edge = F / G
+    ids = rng.permutation(n)
+    at = 0
+    for size, cell in ((3000, (3, 4, 5)), (1500, (7, 7, 7)), (700, (0, G - 1, 2)), (100, (9, 1, 1)),
+                       (65, (5, 9, 3)), (64, (1, 5, 9))):
+        sel = ids[at:at + size]
+        at += size
+        lo = (np.array(cell) + 0.3) * edge
+        st[sel, 0:3] = lo + rng.random((size, 3)) * 0.2 * edge
+    e.set_state(xyzv6=st)
+    check(e, single(e, n, F), f"crowded {E.MODE_NAMES[mode]} t=0")
+    e.step(2)
+    check(e, single(e, n, F), f"crowded {E.MODE_NAMES[mode]} t=2")
+    e.close()

@@ -356,7 +356,7 @@ def test_every_workgroup_size_and_tile_depth():
     for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
         tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
         for block in (64, 128, 256):
-            for tile in (6, 9, 12):
+            for tile in (5, 6, 9, 12):
                 g = B.Engine(B.make_params(n, F, **CFG0, mode=mode, flags=E.FLAG_DIGEST, block=block, tile=tile))
                 info = g.info()
                 assert (info.block, info.tile) == (block, tile)
@@ -368,14 +368,14 @@ def test_every_workgroup_size_and_tile_depth():
                 ep, ev = scaled_err(gp, gv, p, v, F, CFG0["vmax"])
                 assert ep <= tol and ev <= tol, (E.MODE_NAMES[mode], block, tile, ep, ev)
                 g.close()
-        say(f"{E.MODE_NAMES[mode]}: blocks 64/128/256 x tiles 6/9/12 ok")
+        say(f"{E.MODE_NAMES[mode]}: blocks 64/128/256 x tiles 5/6/9/12 ok")
 
 
 def test_clustered_states_neighbour_sets_multi_pass_tiles():
     # Dense clusters: the headline workload after 300 updates (cells up to
     # ~60 boids) and the 64-blob clustered init at the 2^24-boid
     # configuration's density.  Tiles then exceed the LDS capacity and take
-    # several passes (more with tile = 6); neighbour sets must stay exact.
+    # several passes (more with tile = 5); neighbour sets must stay exact.
     n, F = 1 << 20, 4.0
     e = make(n, F, CFG0, E.MODE_GRID_F64)
     e.set_state(xyzv6=iso_state(n, F, 2))
@@ -391,7 +391,7 @@ def test_clustered_states_neighbour_sets_multi_pass_tiles():
         say(f"oracle step ({label}) ...")
         _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
         for mode in (E.MODE_GRID_F64, E.MODE_GRID_MIXED):
-            for tile in (6, 12):
+            for tile in (5, 12):
                 g = B.Engine(B.make_params(n, Fs, **CFG0, mode=mode, flags=E.FLAG_DIGEST, tile=tile))
                 g.set_state(pos=p0, vel=v0)
                 g.step(1)
