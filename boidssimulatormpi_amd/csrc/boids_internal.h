@@ -128,8 +128,8 @@ hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start
 // sorted parts into its in-cell id order).
 hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
                                  const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
-// lat (mixed) receives the lattice copy; rec (FP64 grid) the interleaved
-// records x y z vx vy vz (3 double2 per slot) the neighbour kernel fetches.
+// lat (mixed) receives the lattice copy; rec (FP64 grid) the records the
+// neighbour kernel fetches, three planes of n double2: (x, y), (z, vx), (vy, vz).
 hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,
                          const uint32_t* start, uint32_t* count, StateF64 src, StateF64 dst,
                          uint32_t* key_sorted, Lattice* lat, double2* rec, hipStream_t stream);

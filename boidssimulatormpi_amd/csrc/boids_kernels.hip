@@ -374,10 +374,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* per
         const double iq = P.inv_vq;
         lat.v[f] = make_float4((float)rint(vx * iq), (float)rint(vy * iq), (float)rint(vz * iq), 0.f);
     }
-    if (rec) {
-        rec[3 * (size_t)f] = make_double2(x, y);
-        rec[3 * (size_t)f + 1] = make_double2(z, vx);
-        rec[3 * (size_t)f + 2] = make_double2(vy, vz);
+    if (rec) {                                 // three planes of n records: (x, y), (z, vx), (vy, vz)
+        rec[f] = make_double2(x, y);
+        rec[(size_t)F.n + f] = make_double2(z, vx);
+        rec[2 * (size_t)F.n + f] = make_double2(vy, vz);
     }
 }
 

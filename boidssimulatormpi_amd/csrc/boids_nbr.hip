@@ -122,6 +122,16 @@ __device__ __forceinline__ void stage_tile(const uint32_t* off, const uint32_t* 
     }
 }
 
+// Element i of a 16-byte record array at a wave-uniform base, addressed as
+// base + 32-bit byte offset (16 i < 2^32: engines hold at most 2^28 boids),
+// so a load takes one shift where 64-bit index arithmetic takes three.
+template <typename T>
+__device__ __forceinline__ T at32(const T* base, uint32_t i)
+{
+    static_assert(sizeof(T) == 16, "16-byte records");
+    return *(const T*)((const char*)base + (size_t)(i << 4));
+}
+
 // Value of v in lane src of this wave (all lanes active).
 __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t src)
 {
@@ -318,8 +328,11 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
     LE* list = wl[__builtin_amdgcn_readfirstlane(tid >> 6)];   // wave-uniform: a scalar base
 
     // One (owner, candidate) pair per lane: the candidate's record from L2
-    // by slot, read ahead of the evaluation.
+    // by slot, read ahead of the evaluation.  The records lie in three
+    // planes of FL.n double2: (x, y), (z, vx), (vy, vz).
     struct Rec { double2 a, b, c; uint32_t e, o, j; bool valid; };
+    const double2* rec_b = rec + FL.n;
+    const double2* rec_c = rec + 2 * (size_t)FL.n;
     // sorted slot of a list entry's candidate (pass base `base`): the tile
     // entry (cell) of its tile position gives the cell's slot base
     auto slot_of = [&](uint32_t ent, uint32_t base) {
@@ -332,9 +345,9 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
         r.e = list[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
         r.j = slot_of(r.e, base);
-        r.a = rec[3 * (size_t)r.j];
-        r.b = rec[3 * (size_t)r.j + 1];
-        r.c = rec[3 * (size_t)r.j + 2];
+        r.a = at32(rec, r.j);
+        r.b = at32(rec_b, r.j);
+        r.c = at32(rec_c, r.j);
         return r;
     };
     // The oracle's decisions on the minimum image (src/boid.cpp:27-42,
@@ -601,8 +614,8 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         r.e = lst[min(idx, lim - 1)];
         r.o = wbase + (r.e & 63u);
         r.j = slot_of(r.e, base);
-        r.q = lat.p[r.j];
-        r.w = lat.v[r.j];
+        r.q = at32(lat.p, r.j);
+        r.w = at32(lat.v, r.j);
         return r;
     };
     // FP32 verdicts outside the guard bands, the oracle's FP64 ones inside.
