@@ -99,6 +99,19 @@ __device__ __forceinline__ int32_t coarse_d2(uint2 q, v2s my_xy, uint32_t my_z)
     return __builtin_amdgcn_sdot2(dxy, dxy, dz * dz, true);
 }
 
+// Padding entry of a chunk's tile, far from every owner of the chunk: the
+// owners lie in one (y, z) cell row, whose coarse y and z span 65536 / G
+// units each; the padding sits half the 16-bit range away from the row's
+// centre on both axes, so its coarse distance to an owner is at least
+// sqrt(2) (32768 - 32768 / G) >= 30893 units, above the coarse bound of any
+// radius below the cell edge (at most 65536 / G <= 21846 units).
+__device__ __forceinline__ uint2 far_entry(int cy, int cz, int G)
+{
+    const uint32_t yc = (uint32_t)(2 * cy + 1) * 32768u / (uint32_t)G;
+    const uint32_t zc = (uint32_t)(2 * cz + 1) * 32768u / (uint32_t)G;
+    return make_uint2(((yc + 32768u) & 0xFFFFu) << 16, (zc + 32768u) & 0xFFFFu);
+}
+
 // Stages tile positions [base, base + here); each thread copies one run of
 // consecutive positions, calling STORE(position - base, slot, entry).
 template <int NB, typename STORE>
@@ -203,23 +216,22 @@ template <uint32_t kDrain, typename LE, typename FLUSH>
 __device__ __forceinline__ uint32_t filter_window(const uint2* tco, LE* list, uint32_t k, uint32_t k1,
                                                   uint32_t lane, v2s my_xy, uint32_t my_z, int32_t thr, FLUSH&& flush)
 {
-    auto near = [&](uint2 q) { return coarse_d2(q, my_xy, my_z) <= thr; };
+    // A finished lane tests nothing (its bound is below every distance); a
+    // lane with 1-3 candidates left also tests the next positions up to 3
+    // past its window end: boids outside the window (farther than the
+    // largest radius, rejected exactly by phase 2) or the tile's padding
+    // (far from every owner of the chunk).
     uint32_t tail = 0;
     uint2 a0 = tco[k], a1 = tco[k + 1], a2 = tco[k + 2], a3 = tco[k + 3];
     while (wave_mask(k < k1) != 0) {
         const uint32_t kn = min(k + 4, k1);
         const uint2 b0 = tco[kn], b1 = tco[kn + 1], b2 = tco[kn + 2], b3 = tco[kn + 3];
-        // masks straight from the compares (scalar ANDs); each hit is
-        // stored under its own lane mask
-        const uint32_t left = k1 - k;
-        const bool h0 = (left > 0) & near(a0), h1 = (left > 1) & near(a1);
-        const bool h2 = (left > 2) & near(a2), h3 = (left > 3) & near(a3);
-        const uint64_t m0 = wave_mask(left > 0) & wave_mask(near(a0));
-        const uint64_t m1 = wave_mask(left > 1) & wave_mask(near(a1));
-        const uint64_t m2 = wave_mask(left > 2) & wave_mask(near(a2));
-        const uint64_t m3 = wave_mask(left > 3) & wave_mask(near(a3));
-        const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
+        const int32_t te = k < k1 ? thr : -1;
+        const bool h0 = coarse_d2(a0, my_xy, my_z) <= te, h1 = coarse_d2(a1, my_xy, my_z) <= te;
+        const bool h2 = coarse_d2(a2, my_xy, my_z) <= te, h3 = coarse_d2(a3, my_xy, my_z) <= te;
+        const uint64_t m0 = wave_mask(h0), m1 = wave_mask(h1), m2 = wave_mask(h2), m3 = wave_mask(h3);
         const uint32_t e = (k << 6) | lane;
+        const uint32_t t1 = tail + __popcll(m0), t2 = t1 + __popcll(m1), t3 = t2 + __popcll(m2);
         // the wave-uniform list offsets go into the (scalar) base address
         if (h0) (list + tail)[lanes_below(m0)] = (LE)e;
         if (h1) (list + t1)[lanes_below(m1)] = (LE)(e + (1u << 6));
@@ -407,6 +419,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_f64(Flocks FL, Stat
             const uint2 c = coarse_of(to_lattice(in.x[slot], sc), to_lattice(in.y[slot], sc), to_lattice(in.z[slot], sc));
             tco[kk] = make_uint2(c.x, c.y | (e << 22));
         });
+        if (tid < 4) tco[here + tid] = far_entry(cy, cz, G);   // the look-ahead's padding
         __syncthreads();
         const uint32_t kb = active ? min(max(wlo, base), base + here) : base;
         const uint32_t ke = active ? min(max(whi, kb), base + here) : base;
@@ -685,6 +698,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             const uint2 c = coarse_of(q.x, q.y, q.z);
             tco[kk] = make_uint2(c.x, c.y | (e << 22));
         });
+        if (tid < 4) tco[here + tid] = far_entry(cy, cz, G);   // the look-ahead's padding
         __syncthreads();
         // this lane's window in the staged range [base, base + here) (empty
         // for idle lanes); positions stay <= base + here, so the look-ahead
