@@ -377,24 +377,26 @@ def traffic_from_profiles(mode_name, n):
 
 
 def valu_from_profiles(mode_name, n):
-    """VALU issue fraction and active-lane fraction of the fused kernel from
-    the committed rocprofv3 counter summary (uniform-start launches)."""
+    """VALU issue fraction, active-lane fraction and instruction mix of the
+    fused kernel from the committed rocprofv3 counter summary (every launch of
+    a T = 1000 run)."""
     path = os.path.join(ROOT, "profiles", f"valu_pair_kernels_n{n}.txt")
     if not os.path.exists(path):
         return None
+    keys = ("valu_issue", "active_lanes", "salu_per_valu", "lds_per_valu")
     got = {}
     with open(path) as f:
         for line in f:
-            for key in ("valu_issue", "active_lanes", "valu_issue_t500", "active_lanes_t500"):
+            for key in keys:
                 if line.startswith(f"{key}_{mode_name}:"):
                     got[key] = float(line.split(":", 1)[1])
     if "valu_issue" not in got or "active_lanes" not in got:
         return None
     out = {"issue_frac": got["valu_issue"], "active_lanes": got["active_lanes"],
-           "source": os.path.relpath(path, ROOT), "launches": "updates t = 0..21 (uniform start)"}
-    if "valu_issue_t500" in got and "active_lanes_t500" in got:
-        out["clustered"] = {"issue_frac": got["valu_issue_t500"], "active_lanes": got["active_lanes_t500"],
-                            "launches": "updates t = 500..521"}
+           "source": os.path.relpath(path, ROOT), "launches": "every update of the T = 1000 run"}
+    for key in ("salu_per_valu", "lds_per_valu"):
+        if key in got:
+            out[key] = got[key]
     return out
 
 
@@ -488,9 +490,11 @@ def main():
     if a.steps >= T_run:
         timed = f"the whole {a.steps}-update run from t = {a.from_t}, consecutive updates"
     else:
+        at = ", ".join(str(t + a.from_t) for t in ts) if len(ts) <= 3 else \
+            f"{ts[0] + a.from_t}, {ts[1] + a.from_t}, ..., {ts[-1] + a.from_t}"
         timed = (f"{a.steps} single updates stratified over the {T_run}-update run from t = {a.from_t} "
-                 f"(t = {ts[0] + a.from_t}, {ts[1] + a.from_t}, ..., {ts[-1] + a.from_t}: midpoints of "
-                 f"{a.steps} equal segments), each bracketed by device syncs, the rest untimed")
+                 f"(t = {at}: midpoints of {a.steps} equal segments), each bracketed by device syncs, "
+                 f"the rest untimed")
     dtype = {"grid_mixed": "f32+f64", "allpairs_f32ref": "f32"}.get(head["mode"], "f64")
     out = {
         "metric": metric,

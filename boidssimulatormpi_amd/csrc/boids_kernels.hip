@@ -17,21 +17,40 @@ namespace boids {
 
 
 // ---------------------------------------------------------------------------
-// All-pairs step: one thread per boid i, candidates j = 0..n-1 ascending,
-// staged through LDS tiles.  Same per-pair arithmetic and summation order
-// as the reference (src/boid_simulation.cpp:153-248), fused with phase B.
-// R = float: bit-identical to boidsim.  R = double: the FP64 promotion.
+// All-pairs step: one wave per boid i.  The 64 lanes take the decisions for
+// candidates j = base + lane (the reference's per-pair arithmetic,
+// src/boid_simulation.cpp:153-212), then the wave adds the in-range pairs'
+// terms in ascending j -- the reference's summation order -- one pair at a
+// time, every lane holding the same sums, and lane 0 runs phase B
+// (:216-248).  R = float: bit-identical to boidsim.  R = double: the FP64
+// promotion.  A flock of N boids takes N waves, so small flocks (cfg0's
+// N = 1024) still fill the chip; the per-pair decisions, not the ordered
+// sums, are the work.
 // ---------------------------------------------------------------------------
-template <typename R, typename St>
-__global__ __launch_bounds__(kBlock) void k_allpairs(DevParams P, St in, St out, R* forces,
-                                                     uint64_t* digest)
+constexpr int kWaveOwners = kBlock / 64;     // owners (waves) per workgroup
+
+template <typename R>
+__device__ __forceinline__ R lane_value(R v, uint32_t l)
 {
-    __shared__ R tx[kBlock], ty[kBlock], tz[kBlock], tvx[kBlock], tvy[kBlock], tvz[kBlock];
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)l));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)l);
+        return __builtin_bit_cast(R, ((uint64_t)hi << 32) | lo);
+    }
+}
+
+template <typename R, typename St>
+__global__ __launch_bounds__(kBlock) void k_allpairs_wave(DevParams P, St in, St out, R* forces,
+                                                          uint64_t* digest)
+{
     const uint32_t n = P.n;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = i < n;
-    R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    if (active) { px = in.x[i]; py = in.y[i]; pz = in.z[i]; vx = in.vx[i]; vy = in.vy[i]; vz = in.vz[i]; }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t i = blockIdx.x * kWaveOwners + (threadIdx.x >> 6);
+    if (i >= n) return;                                  // the whole wave
+    R px = in.x[i], py = in.y[i], pz = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
     const R vn = norm3(vx, vy, vz);
     const R smax = Th<R>::smax(P);
     const R s_c = Th<R>::s(P, COH), s_s = Th<R>::s(P, SEP), s_a = Th<R>::s(P, ALI);
@@ -41,42 +60,52 @@ __global__ __launch_bounds__(kBlock) void k_allpairs(DevParams P, St in, St out,
     int nc = 0, ns = 0, na = 0;
     uint64_t hc = 0, hs = 0, ha = 0;
 
-    for (uint32_t base = 0; base < n; base += kBlock) {
-        __syncthreads();
-        const uint32_t j = base + threadIdx.x;
-        if (j < n) {
-            tx[threadIdx.x] = in.x[j]; ty[threadIdx.x] = in.y[j]; tz[threadIdx.x] = in.z[j];
-            tvx[threadIdx.x] = in.vx[j]; tvy[threadIdx.x] = in.vy[j]; tvz[threadIdx.x] = in.vz[j];
-        }
-        __syncthreads();
-        const uint32_t m = min((uint32_t)kBlock, n - base);
-        if (!active) continue;
-        for (uint32_t k = 0; k < m; ++k) {
-            const R qx = min_image(tx[k], px, P.F, P.half);
-            const R qy = min_image(ty[k], py, P.F, P.half);
-            const R qz = min_image(tz[k], pz, P.F, P.half);
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t j = base + lane;
+        bool in_c = false, in_s = false, in_a = false;
+        R qx = 0, qy = 0, qz = 0, tx = 0, ty = 0, tz = 0, wx = 0, wy = 0, wz = 0;
+        if (j < n && j != i) {
+            qx = min_image(in.x[j], px, P.F, P.half);
+            qy = min_image(in.y[j], py, P.F, P.half);
+            qz = min_image(in.z[j], pz, P.F, P.half);
             const R ex = px - qx, ey = py - qy, ez = pz - qz;
             const R s = ex * ex + ey * ey + ez * ez;
-            if (!(s < smax)) continue;
-            if (base + k == i) continue;
-            bool in_c = s < s_c, in_s = s < s_s, in_a = s < s_a;
-            if ((in_c && k_c) || (in_s && k_s) || (in_a && k_a)) {
-                const R gx = qx - px, gy = qy - py, gz = qz - pz;
-                const R dot = vx * gx + vy * gy + vz * gz;
-                const R c = dot / (vn * sqrt(s));
-                if (k_c) in_c = in_c && c >= c_c && c <= (R)1;
-                if (k_s) in_s = in_s && c >= c_s && c <= (R)1;
-                if (k_a) in_a = in_a && c >= c_a && c <= (R)1;
+            if (s < smax) {
+                in_c = s < s_c; in_s = s < s_s; in_a = s < s_a;
+                if ((in_c && k_c) || (in_s && k_s) || (in_a && k_a)) {
+                    const R gx = qx - px, gy = qy - py, gz = qz - pz;
+                    const R dot = vx * gx + vy * gy + vz * gz;
+                    const R c = dot / (vn * sqrt(s));
+                    if (k_c) in_c = in_c && c >= c_c && c <= (R)1;
+                    if (k_s) in_s = in_s && c >= c_s && c <= (R)1;
+                    if (k_a) in_a = in_a && c >= c_a && c <= (R)1;
+                }
+                if (in_s) {
+                    const R t = sqrt(s);
+                    tx = ex / t; ty = ey / t; tz = ez / t;
+                }
+                if (in_a) { wx = in.vx[j]; wy = in.vy[j]; wz = in.vz[j]; }
             }
-            if (in_c) { nc++; scx += qx; scy += qy; scz += qz; hc += mix64(base + k); }
-            if (in_s) {
-                const R t = sqrt(s);
-                ns++; ssx += ex / t; ssy += ey / t; ssz += ez / t; hs += mix64(base + k);
+        }
+        const int flags = (in_c ? 1 : 0) | (in_s ? 2 : 0) | (in_a ? 4 : 0);
+        uint64_t m = __builtin_amdgcn_ballot_w64(flags != 0);
+        while (m) {                                      // ascending j: the reference's order
+            const uint32_t l = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const int f = __builtin_amdgcn_readlane(flags, (int)l);
+            const uint64_t h = mix64(base + l);
+            if (f & 1) {
+                nc++; scx += lane_value(qx, l); scy += lane_value(qy, l); scz += lane_value(qz, l); hc += h;
             }
-            if (in_a) { na++; sax += tvx[k]; say += tvy[k]; saz += tvz[k]; ha += mix64(base + k); }
+            if (f & 2) {
+                ns++; ssx += lane_value(tx, l); ssy += lane_value(ty, l); ssz += lane_value(tz, l); hs += h;
+            }
+            if (f & 4) {
+                na++; sax += lane_value(wx, l); say += lane_value(wy, l); saz += lane_value(wz, l); ha += h;
+            }
         }
     }
-    if (!active) return;
+    if (lane != 0) return;
     if (nc != 0) { const R k = (R)nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
     if (na != 0) { const R k = (R)na; sax = sax / k - vx; say = say / k - vy; saz = saz / k - vz; }
     const R wc = Th<R>::w(P, COH), ws = Th<R>::w(P, SEP), wa = Th<R>::w(P, ALI);
@@ -84,8 +113,9 @@ __global__ __launch_bounds__(kBlock) void k_allpairs(DevParams P, St in, St out,
     const R dy = (scy * wc + ssy * ws) + say * wa;
     const R dz = (scz * wc + ssz * ws) + saz * wa;
     if (forces) {
-        R* f = forces + 9 * (size_t)i;
-        f[0] = scx; f[1] = scy; f[2] = scz; f[3] = ssx; f[4] = ssy; f[5] = ssz; f[6] = sax; f[7] = say; f[8] = saz;
+        R* fo = forces + 9 * (size_t)i;
+        fo[0] = scx; fo[1] = scy; fo[2] = scz; fo[3] = ssx; fo[4] = ssy; fo[5] = ssz;
+        fo[6] = sax; fo[7] = say; fo[8] = saz;
     }
     if (digest) {
         uint64_t* d = digest + 4 * (size_t)i;
@@ -399,8 +429,8 @@ hipError_t launch_allpairs_f32ref(const DevParams& P, StateF32 in, StateF32 out,
                                   uint64_t* digest, hipStream_t stream)
 {
     if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_allpairs<float, StateF32>), dim3(blocks_for(P.n)), dim3(kBlock), 0, stream,
-                       P, in, out, forces, digest);
+    hipLaunchKernelGGL((k_allpairs_wave<float, StateF32>), dim3((P.n + kWaveOwners - 1) / kWaveOwners),
+                       dim3(kBlock), 0, stream, P, in, out, forces, digest);
     return hipGetLastError();
 }
 
@@ -408,8 +438,8 @@ hipError_t launch_allpairs_f64(const DevParams& P, StateF64 in, StateF64 out, do
                                uint64_t* digest, hipStream_t stream)
 {
     if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_allpairs<double, StateF64>), dim3(blocks_for(P.n)), dim3(kBlock), 0, stream,
-                       P, in, out, forces, digest);
+    hipLaunchKernelGGL((k_allpairs_wave<double, StateF64>), dim3((P.n + kWaveOwners - 1) / kWaveOwners),
+                       dim3(kBlock), 0, stream, P, in, out, forces, digest);
     return hipGetLastError();
 }
 
