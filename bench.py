@@ -290,8 +290,9 @@ def _ref_update_seconds(exe, n, field, init, seed, threads, budget_s):
         subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL, timeout=900)
         return time.perf_counter() - t0
     w1 = run(1)
-    per = max(run(2) - w1, 1e-4)                         # one update, for sizing
-    k = max(2, int(budget_s / per))
+    per = max(run(5) / 5, 1e-4)                          # for sizing only: start-up included, never an underestimate
+    k = min(max(4, int(budget_s / per)), 100000)
+    log(f"[bench]   {os.path.basename(exe)}, {threads} thread(s): {k} updates (~{k * per:.0f} s)")
     wk = run(1 + k)
     return (wk - w1) / k, k
 
