@@ -142,7 +142,8 @@ __device__ __forceinline__ uint32_t to_lattice(double x, double scale)
 // Sub-lattice residuals: x * scale - rint(x * scale) lies in [-1/2, 1/2]
 // lattice units; rounded to kSubBits fractional bits it fits a signed
 // 10-bit field ([-256, 256]).  Three fields pack into the lattice record's
-// spare word (x in bits 0-9, y in 10-19, z in 20-29), so a displacement
+// spare word (x in bits 0-9, y in 10-19, z in 20-29; bit 31 marks a boid
+// whose position or velocity is NaN), so a displacement
 // refined to 2^-kSubBits units costs no extra bytes.  It matters where a
 // term is normalised: the separation unit vector e/|e| of a close pair
 // would otherwise carry the 1/2-unit rounding of both ends.

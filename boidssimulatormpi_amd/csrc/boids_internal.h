@@ -66,7 +66,7 @@ struct DevParams {
     int32_t coarse_thr;        // squared distance bound on the top 16 lattice bits
     // velocity quantum of the mixed lattice records: a power of two with
     // |v| / vq < 2^31 for |v| <= vmax, so quantised velocities are integers
-    // and their FP64 sums are exact in any order
+    // and their sums are exact in any order
     double vq, inv_vq;
 };
 
@@ -80,7 +80,7 @@ struct StateF32 {
 };
 
 struct Lattice {               // mixed mode companion of the sorted FP64 state
-    uint4* p;                  // x, y, z lattice coordinates; w: packed sub-lattice residuals
+    uint4* p;                  // x, y, z lattice coordinates; w: packed sub-lattice residuals (bit 31: NaN state)
     float4* v;                 // velocity in quanta of DevParams::vq, integer-valued FP32 (w unused)
 };
 

@@ -399,8 +399,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Flocks F, const uint32_t* per
     if (with_lat) {
         const DevParams& P = F.fl[flock_of(F, f)].P;   // slots stay inside their flock
         const double scale = P.lat_scale;
+        const bool nan_state = x != x || y != y || z != z || vx != vx || vy != vy || vz != vz;
+        const uint32_t nanv = nan_state ? 0x80000000u : 0u;   // NaN state mark
         lat.p[f] = make_uint4(to_lattice(x, scale), to_lattice(y, scale), to_lattice(z, scale),
-                              pack_sub(x, y, z, scale));
+                              pack_sub(x, y, z, scale) | nanv);
         const double iq = P.inv_vq;
         lat.v[f] = make_float4((float)rint(vx * iq), (float)rint(vy * iq), (float)rint(vz * iq), 0.f);
     }

@@ -480,6 +480,22 @@ __device__ __forceinline__ void rule_bands(const DevParams& P, int r, float s2, 
 constexpr double kSepQ = 1073741824.0;                // separation terms in quanta of 2^-30
 constexpr double kSepQInv = 1.0 / 1073741824.0;
 
+// Adds a 32-bit integer term to a 64-bit LDS sum: two's-complement sums are
+// exact and independent of the order terms arrive in, with an integer
+// atomic and no conversion to double per term.
+__device__ __forceinline__ void add_i64(int64_t& sum, int32_t v)
+{
+    atomicAdd((unsigned long long*)&sum, (unsigned long long)(int64_t)v);
+}
+
+// A boid's neighbour counts share one 64-bit LDS word, nc | ns << 21 |
+// na << 42 (the digest's layout).  Counts stay below 2^20, so bit 41 is
+// free: it marks a separation term that is NaN -- the reference's 0/0 unit
+// vector of a coincident pair -- which an integer sum cannot carry; the
+// owner's separation sum is then NaN, as the reference's is.
+constexpr uint64_t kNanSep = 1ull << 41;
+constexpr uint64_t kCounts = ~kNanSep;
+
 template <int NB, int TD, bool DIGEST>
 __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, StateF64 in, StateF64 out,
                                                       Lattice lat,
@@ -502,7 +518,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     // it, the tile's first words serve as the tile-table scan's scratch.
     __shared__ struct {
         uint2 tco[Cfg::kCap + 4];
-        double acc[9][NB];                      // integer-valued sums: cohesion, separation, alignment
+        int64_t acc[9][NB];                     // exact integer sums: cohesion, separation, alignment
     } sh;
     auto& tco = sh.tco;
     auto& acc = sh.acc;
@@ -557,7 +573,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
     const uint32_t my_z = mc.y;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q][tid] = 0.0;
+    for (int q = 0; q < 9; ++q) acc[q][tid] = 0;
     acc_n[tid] = 0;
     if (DIGEST) { acc_h[tid] = 0; acc_h[NB + tid] = 0; acc_h[2 * NB + tid] = 0; }
     uint32_t nre = 0;
@@ -582,24 +598,27 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         double y = __builtin_amdgcn_rsq(e2);           // estimate, then one Newton step
         y = y * fma(-0.5 * e2, y * y, 1.5);
         const double t = -y * kSepQ;
-        atomicAdd(&acc[3][o], rint(ex * t));
-        atomicAdd(&acc[4][o], rint(ey * t));
-        atomicAdd(&acc[5][o], rint(ez * t));
+        // |e/|e|| * 2^30 <= 2^30; a coincident pair (e = 0) adds 0 and marks
+        // the owner's separation sum NaN
+        add_i64(acc[3][o], (int32_t)rint(ex * t));
+        add_i64(acc[4][o], (int32_t)rint(ey * t));
+        add_i64(acc[5][o], (int32_t)rint(ez * t));
+        if (!(e2 > 0.0)) atomicOr((unsigned long long*)&acc_n[o], (unsigned long long)kNanSep);
     };
     // Adds the pair (owner o at lattice m, candidate record q, w, slot j)
-    // to the owner's sums.  Every term is an integer-valued double, so the
-    // LDS sums are exact and independent of the order pairs arrive in:
+    // to the owner's sums.  Every term is an integer, so the 64-bit LDS
+    // sums are exact and independent of the order pairs arrive in:
     // cohesion adds lattice displacements, alignment quantised velocities
     // (lattice record), separation the quantised unit vector.
     auto accumulate = [&](uint32_t o, const uint4& m, const uint4& q, const float4& w, uint32_t j, bool in_c,
                           bool in_s, bool in_a) {
         const int32_t ix = (int32_t)(q.x - m.x), iy = (int32_t)(q.y - m.y), iz = (int32_t)(q.z - m.z);
         if (in_c) {
-            atomicAdd(&acc[0][o], (double)ix); atomicAdd(&acc[1][o], (double)iy); atomicAdd(&acc[2][o], (double)iz);
+            add_i64(acc[0][o], ix); add_i64(acc[1][o], iy); add_i64(acc[2][o], iz);
         }
         if (in_s) sep_term(o, m, q);
         if (in_a) {
-            atomicAdd(&acc[6][o], (double)w.x); atomicAdd(&acc[7][o], (double)w.y); atomicAdd(&acc[8][o], (double)w.z);
+            add_i64(acc[6][o], (int32_t)w.x); add_i64(acc[7][o], (int32_t)w.y); add_i64(acc[8][o], (int32_t)w.z);
         }
         if (in_c | in_s | in_a)
             atomicAdd(&acc_n[o], (uint64_t)in_c | ((uint64_t)in_s << 21) | ((uint64_t)in_a << 42));
@@ -644,7 +663,10 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         const float c = fmaf(r.ov.z, gz, fmaf(r.ov.y, gy, r.ov.x * gx)) * inv_d * r.ov.w;
         const float band = P.band_k * inv_d;
         const float c_dn = c - band, c_up = c + band;
-        const bool skip = !r.valid | (r.j == t0 + r.o);    // padding, or the owner itself
+        // padding, the owner itself, or a boid whose state is NaN (bit 31 of
+        // the residual word: its lattice coordinates mean nothing, and the
+        // reference's NaN distance makes it nobody's neighbour)
+        const bool skip = !r.valid | (r.j == t0 + r.o) | ((int32_t)(r.q.w | r.m.w) < 0);
         bool in_c, out_c, in_s, out_s, in_a, out_a;
         if (P.same_cone) {
             // one view cone for all three rules (a flock-uniform branch): its
@@ -727,25 +749,30 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     const double vx = in.vx[s], vy = in.vy[s], vz = in.vz[s];
     const uint32_t myid = in.id[s];
     // this wave's atomics are complete: LDS operations of a wave execute in order
-    const uint64_t cnt = acc_n[tid];
+    const uint64_t word = acc_n[tid], cnt = word & kCounts;
     const uint32_t nc = (uint32_t)(cnt & 0x1FFFFFu), na = (uint32_t)(cnt >> 42);
+    const double qnan = __builtin_nan("");
     // dv_coh = mean(q) - p = mean(q - p): the lattice sums are already relative
     double dcx = 0, dcy = 0, dcz = 0, dax = 0, day = 0, daz = 0;
     if (nc != 0) {
         const double k = P.unit / (double)nc;
-        dcx = acc[0][tid] * k; dcy = acc[1][tid] * k; dcz = acc[2][tid] * k;
+        dcx = (double)acc[0][tid] * k; dcy = (double)acc[1][tid] * k; dcz = (double)acc[2][tid] * k;
     }
     if (na != 0) {
         const double k = (double)na;
-        dax = acc[6][tid] * P.vq / k - vx; day = acc[7][tid] * P.vq / k - vy; daz = acc[8][tid] * P.vq / k - vz;
+        dax = (double)acc[6][tid] * P.vq / k - vx; day = (double)acc[7][tid] * P.vq / k - vy;
+        daz = (double)acc[8][tid] * P.vq / k - vz;
     }
+    double dsx = (double)acc[3][tid] * kSepQInv, dsy = (double)acc[4][tid] * kSepQInv;
+    double dsz = (double)acc[5][tid] * kSepQInv;
+    if (word & kNanSep) dsx = dsy = dsz = qnan;
     if (DIGEST) {
         uint64_t* d = digest + 4 * (size_t)myid;
         d[0] = cnt;
         d[1] = acc_h[tid]; d[2] = acc_h[NB + tid]; d[3] = acc_h[2 * NB + tid];
     }
-    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, acc[3][tid] * kSepQInv,
-                acc[4][tid] * kSepQInv, acc[5][tid] * kSepQInv, dax, day, daz, out, key_out, rank, count, forces);
+    finish_boid(P, cbase, s, myid, px, py, pz, vx, vy, vz, dcx, dcy, dcz, dsx, dsy, dsz, dax, day, daz, out, key_out,
+                rank, count, forces);
 }
 
 // ---------------------------------------------------------------------------

@@ -267,15 +267,20 @@ def test_rules_without_view_cone(mode, name):
                    [0.51, 0.5, 0.5, 0.0, 0.0, 0.0], [0.995, 0.5, 0.5, 0.002, 0.001, 0]], np.float32)
     e = make(4, F, rules, mode, E.FLAG_DIGEST)
     e.set_state(xyzv6=st)
-    e.step(1)
     p, v = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
-    _, d = O.step(P, p, v, digest=True)
-    assert np.array_equal(e.digest(), d)
-    gp, gv = e.state_f64()
-    assert np.array_equal(np.isnan(gp), np.isnan(p)) and np.array_equal(np.isnan(gv), np.isnan(v))
-    ok = ~(np.isnan(p).any(1) | np.isnan(v).any(1))
-    ep, ev = scaled_err(gp[ok], gv[ok], p[ok], v[ok], F, rules["vmax"])
-    assert ep < 1e-9 and ev < 1e-9
+    # two updates: the second starts from NaN states, whose NaN distances make
+    # them nobody's neighbours in the reference, so the NaN must not spread
+    for t in range(2):
+        e.step(1)
+        _, d = O.step(P, p, v, digest=True)
+        assert np.array_equal(e.digest(), d), t
+        gp, gv = e.state_f64()
+        assert np.array_equal(np.isnan(gp), np.isnan(p)) and np.array_equal(np.isnan(gv), np.isnan(v)), t
+        ok = ~(np.isnan(p).any(1) | np.isnan(v).any(1))
+        ep, ev = scaled_err(gp[ok], gv[ok], p[ok], v[ok], F, rules["vmax"])
+        # the second update carries the first one's mixed-precision error
+        tol = 1e-9 if t == 0 or mode == E.MODE_GRID_F64 else 1e-6
+        assert ep < tol and ev < tol, (t, ep, ev)
     say(f"{name} {E.MODE_NAMES[mode]} ok")
 
 
