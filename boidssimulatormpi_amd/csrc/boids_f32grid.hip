@@ -94,36 +94,28 @@ __global__ __launch_bounds__(kBlock) void k_f32_cells(uint32_t n, StateF32 st, c
     vrec[f] = make_float4(st.vx[me], st.vy[me], st.vz[me], 0.f);
 }
 
-// One owner per thread (id order).  Per neighbour cell k of the 27: cur[k]
-// = the cell's next unvisited slot, end[k] its end, head[k] = the id at
-// cur[k] once cur[k] has been moved to the next in-range candidate (~0u:
-// the cell is exhausted).  Each merge step takes the smallest head, adds
-// that boid to the sums (the reference's order), and moves its cell on.
-constexpr int kF32GridBlock = 64;
+// Sums of one owner in the reference's order, the update's outputs.
+struct SumsF32 {
+    float scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
+    int nc = 0, ns = 0, na = 0;
+    uint64_t hc = 0, hs = 0, ha = 0;
+};
 
-template <int NB>
-__global__ __launch_bounds__(NB) void k_f32_grid(DevParams P, StateF32 in, StateF32 out,
-                                                 const float4* __restrict__ rec, const float4* __restrict__ vrec,
-                                                 const uint32_t* __restrict__ key,
-                                                 const uint32_t* __restrict__ start, float* forces,
-                                                 uint64_t* digest)
+// Serial form: the owner's in-range boids visited in ascending id by a lazy
+// 27-way merge.  Per neighbour cell k: cur[k] = the cell's next unvisited
+// slot, endp[k] its end, head[k] = the id at cur[k] once cur[k] has been
+// moved to the next in-range candidate (~0u: the cell is exhausted).  Each
+// merge step takes the smallest head, adds that boid to the sums and moves
+// its cell on.  cur/endp/head: 27 words each of scratch.
+__device__ SumsF32 merge_serial(const DevParams& P, const float4* __restrict__ rec, const float4* __restrict__ vrec,
+                                const uint32_t* __restrict__ start, uint32_t i, float px, float py, float pz,
+                                float vx, float vy, float vz, float vn, int cx, int cy, int cz, uint32_t* cur,
+                                uint32_t* endp, uint32_t* head)
 {
-    __shared__ uint32_t cur[27][NB], endp[27][NB], head[27][NB];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t i = blockIdx.x * NB + tid;
-    if (i >= P.n) return;                          // no barrier below: every thread is on its own
-    const float px = in.x[i], py = in.y[i], pz = in.z[i];
-    const float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-    const float vn = norm3(vx, vy, vz);
     const int G = P.G;
-    const uint32_t c = key[i];
-    const int cx = (int)(c % (uint32_t)G), cy = (int)((c / (uint32_t)G) % (uint32_t)G),
-              cz = (int)(c / ((uint32_t)G * (uint32_t)G));
-
-    // moves cell k's cursor to its next in-range candidate (not the owner)
     auto advance = [&](int k) {
-        uint32_t u = cur[k][tid];
-        const uint32_t e = endp[k][tid];
+        uint32_t u = cur[k];
+        const uint32_t e = endp[k];
         uint32_t h = ~0u;
         for (; u < e; ++u) {
             const float4 q = rec[u];
@@ -132,8 +124,8 @@ __global__ __launch_bounds__(NB) void k_f32_grid(DevParams P, StateF32 in, State
             const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
             if (r.in_c | r.in_s | r.in_a) { h = j; break; }
         }
-        cur[k][tid] = u;
-        head[k][tid] = h;
+        cur[k] = u;
+        head[k] = h;
     };
     for (int k = 0; k < 27; ++k) {
         int x = cx + k % 3 - 1, y = cy + (k / 3) % 3 - 1, z = cz + k / 9 - 1;
@@ -141,51 +133,202 @@ __global__ __launch_bounds__(NB) void k_f32_grid(DevParams P, StateF32 in, State
         y += y < 0 ? G : (y >= G ? -G : 0);
         z += z < 0 ? G : (z >= G ? -G : 0);
         const uint32_t cc = (uint32_t)x + (uint32_t)G * ((uint32_t)y + (uint32_t)G * (uint32_t)z);
-        cur[k][tid] = start[cc];
-        endp[k][tid] = start[cc + 1];
+        cur[k] = start[cc];
+        endp[k] = start[cc + 1];
         advance(k);
     }
-    float scx = 0, scy = 0, scz = 0, ssx = 0, ssy = 0, ssz = 0, sax = 0, say = 0, saz = 0;
-    int nc = 0, ns = 0, na = 0;
-    uint64_t hc = 0, hs = 0, ha = 0;
+    SumsF32 S;
     for (;;) {
         uint32_t best = ~0u;
         int kb = 0;
         for (int k = 0; k < 27; ++k) {
-            const uint32_t h = head[k][tid];
+            const uint32_t h = head[k];
             if (h < best) { best = h; kb = k; }
         }
         if (best == ~0u) break;
-        const uint32_t u = cur[kb][tid];
+        const uint32_t u = cur[kb];
         const float4 q = rec[u];
         const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
-        if (r.in_c) { nc++; scx += r.qx; scy += r.qy; scz += r.qz; hc += mix64(best); }
+        if (r.in_c) { S.nc++; S.scx += r.qx; S.scy += r.qy; S.scz += r.qz; S.hc += mix64(best); }
         if (r.in_s) {
             const float t = sqrtf(r.s);
-            ns++; ssx += r.ex / t; ssy += r.ey / t; ssz += r.ez / t; hs += mix64(best);
+            S.ns++; S.ssx += r.ex / t; S.ssy += r.ey / t; S.ssz += r.ez / t; S.hs += mix64(best);
         }
         if (r.in_a) {
             const float4 w = vrec[u];
-            na++; sax += w.x; say += w.y; saz += w.z; ha += mix64(best);
+            S.na++; S.sax += w.x; S.say += w.y; S.saz += w.z; S.ha += mix64(best);
         }
-        cur[kb][tid] = u + 1;
+        cur[kb] = u + 1;
         advance(kb);
     }
+    return S;
+}
+
+// Wave form: one wave per owner (id order).  (1) The lanes take the
+// decisions for the candidates of the 27 cells, 64 at a time in (cell,
+// slot) order, and append the in-range ones to a per-wave LDS list as
+// (id << 32 | slot).  (2) The list is sorted by id (bitonic network in LDS,
+// one wave).  (3) The lanes compute 64 sorted pairs' terms at a time and
+// the wave adds them in ascending id, one pair at a time, every lane
+// holding the same sums: the reference's order.  An owner with more
+// in-range boids than the list holds takes the serial merge on lane 0.
+constexpr int kF32WaveOwners = 4;            // owners (waves) per workgroup
+constexpr uint32_t kF32HitCap = 1024;        // in-range boids per owner sorted in LDS
+
+// Orders this wave's LDS writes before its later LDS reads (a wave's LDS
+// operations execute in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_order()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_of(T v, uint32_t l)
+{
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)l));
+}
+
+__global__ __launch_bounds__(kF32WaveOwners * 64) void k_f32_grid_wave(DevParams P, StateF32 in, StateF32 out,
+                                                                      const float4* __restrict__ rec,
+                                                                      const float4* __restrict__ vrec,
+                                                                      const uint32_t* __restrict__ key,
+                                                                      const uint32_t* __restrict__ start,
+                                                                      float* forces, uint64_t* digest)
+{
+    __shared__ uint64_t hits[kF32WaveOwners][kF32HitCap];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * kF32WaveOwners + w;
+    if (i >= P.n) return;                                // the whole wave; no workgroup barrier below
+    uint64_t* hb = hits[w];
+    const float px = in.x[i], py = in.y[i], pz = in.z[i];
+    const float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+    const float vn = norm3(vx, vy, vz);
+    const int G = P.G;
+    const uint32_t c = key[i];
+    const int cx = (int)(c % (uint32_t)G), cy = (int)((c / (uint32_t)G) % (uint32_t)G),
+              cz = (int)(c / ((uint32_t)G * (uint32_t)G));
+
+    // lane k < 27: neighbour cell k's slot range; exclusive prefix of the sizes
+    uint32_t ca = 0, cn = 0;
+    if (lane < 27) {
+        const int k = (int)lane;
+        int x = cx + k % 3 - 1, y = cy + (k / 3) % 3 - 1, z = cz + k / 9 - 1;
+        x += x < 0 ? G : (x >= G ? -G : 0);
+        y += y < 0 ? G : (y >= G ? -G : 0);
+        z += z < 0 ? G : (z >= G ? -G : 0);
+        const uint32_t cc = (uint32_t)x + (uint32_t)G * ((uint32_t)y + (uint32_t)G * (uint32_t)z);
+        ca = start[cc];
+        cn = start[cc + 1] - ca;
+    }
+    uint32_t incl = cn;
+    for (uint32_t d = 1; d < 32; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t excl = incl - cn;
+    const uint32_t M = (uint32_t)__builtin_amdgcn_readlane((int)incl, 26);
+
+    // (1) decisions, hits appended in (cell, slot) order
+    uint32_t H = 0;
+    for (uint32_t m0 = 0; m0 < M; m0 += 64) {
+        const uint32_t m = min(m0 + lane, M - 1);
+        uint32_t kk = 0;
+        for (uint32_t k = 1; k < 27; ++k)
+            kk = m >= (uint32_t)__builtin_amdgcn_readlane((int)excl, (int)k) ? k : kk;
+        const uint32_t slot = (uint32_t)__shfl((int)ca, (int)kk, 64) + (m - (uint32_t)__shfl((int)excl, (int)kk, 64));
+        bool hit = false;
+        uint32_t j = 0;
+        if (m0 + lane < M) {
+            const float4 q = rec[slot];
+            j = __builtin_bit_cast(uint32_t, q.w);
+            if (j != i) {
+                const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
+                hit = r.in_c | r.in_s | r.in_a;
+            }
+        }
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
+        const uint32_t pos = H + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (hit && pos < kF32HitCap) hb[pos] = ((uint64_t)j << 32) | slot;
+        H += (uint32_t)__popcll(mask);
+    }
+
+    SumsF32 S;
+    if (H > kF32HitCap) {
+        // more in-range boids than the list holds: the serial merge (lane 0)
+        if (lane != 0) return;
+        uint32_t* scratch = (uint32_t*)hb;
+        S = merge_serial(P, rec, vrec, start, i, px, py, pz, vx, vy, vz, vn, cx, cy, cz, scratch, scratch + 27,
+                         scratch + 54);
+    } else if (H > 0) {
+        // (2) bitonic sort of the list by id, padded to a power of two
+        uint32_t P2 = 1;
+        while (P2 < H) P2 <<= 1;
+        for (uint32_t t = H + lane; t < P2; t += 64) hb[t] = ~0ull;
+        wave_lds_order();
+        for (uint32_t size = 2; size <= P2; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = lane; t < P2 / 2; t += 64) {
+                    const uint32_t a = 2 * t - (t & (stride - 1)), b = a + stride;
+                    const uint64_t ka = hb[a], kb = hb[b];
+                    if ((ka > kb) == ((a & size) == 0)) { hb[a] = kb; hb[b] = ka; }
+                }
+                wave_lds_order();
+            }
+        }
+        // (3) terms of 64 sorted pairs at a time, added in ascending id
+        for (uint32_t h0 = 0; h0 < H; h0 += 64) {
+            const uint32_t h = h0 + lane;
+            int flags = 0;
+            float qx = 0, qy = 0, qz = 0, tx = 0, ty = 0, tz = 0, wx = 0, wy = 0, wz = 0;
+            uint32_t j = 0;
+            if (h < H) {
+                const uint64_t e = hb[h];
+                const uint32_t slot = (uint32_t)e;
+                j = (uint32_t)(e >> 32);
+                const float4 q = rec[slot];
+                const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
+                flags = (r.in_c ? 1 : 0) | (r.in_s ? 2 : 0) | (r.in_a ? 4 : 0);
+                qx = r.qx; qy = r.qy; qz = r.qz;
+                if (r.in_s) {
+                    const float t = sqrtf(r.s);
+                    tx = r.ex / t; ty = r.ey / t; tz = r.ez / t;
+                }
+                if (r.in_a) {
+                    const float4 v = vrec[slot];
+                    wx = v.x; wy = v.y; wz = v.z;
+                }
+            }
+            const uint32_t cnt = min(64u, H - h0);
+            for (uint32_t l = 0; l < cnt; ++l) {             // ascending id: the reference's order
+                const int f = __builtin_amdgcn_readlane(flags, (int)l);
+                const uint64_t hh = mix64((uint32_t)__builtin_amdgcn_readlane((int)j, (int)l));
+                if (f & 1) { S.nc++; S.scx += lane_of(qx, l); S.scy += lane_of(qy, l); S.scz += lane_of(qz, l); S.hc += hh; }
+                if (f & 2) { S.ns++; S.ssx += lane_of(tx, l); S.ssy += lane_of(ty, l); S.ssz += lane_of(tz, l); S.hs += hh; }
+                if (f & 4) { S.na++; S.sax += lane_of(wx, l); S.say += lane_of(wy, l); S.saz += lane_of(wz, l); S.ha += hh; }
+            }
+        }
+        if (lane != 0) return;
+    } else if (lane != 0) {
+        return;
+    }
     float ox = px, oy = py, oz = pz, ovx = vx, ovy = vy, ovz = vz;
-    if (nc != 0) { const float k = (float)nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
-    if (na != 0) { const float k = (float)na; sax = sax / k - vx; say = say / k - vy; saz = saz / k - vz; }
+    float scx = S.scx, scy = S.scy, scz = S.scz, sax = S.sax, say = S.say, saz = S.saz;
+    if (S.nc != 0) { const float k = (float)S.nc; scx = scx / k - px; scy = scy / k - py; scz = scz / k - pz; }
+    if (S.na != 0) { const float k = (float)S.na; sax = sax / k - vx; say = say / k - vy; saz = saz / k - vz; }
     const float wc = P.w_f[COH], ws = P.w_f[SEP], wa = P.w_f[ALI];
-    const float dx = (scx * wc + ssx * ws) + sax * wa;
-    const float dy = (scy * wc + ssy * ws) + say * wa;
-    const float dz = (scz * wc + ssz * ws) + saz * wa;
+    const float dx = (scx * wc + S.ssx * ws) + sax * wa;
+    const float dy = (scy * wc + S.ssy * ws) + say * wa;
+    const float dz = (scz * wc + S.ssz * ws) + saz * wa;
     if (forces) {
         float* f = forces + 9 * (size_t)i;
-        f[0] = scx; f[1] = scy; f[2] = scz; f[3] = ssx; f[4] = ssy; f[5] = ssz; f[6] = sax; f[7] = say; f[8] = saz;
+        f[0] = scx; f[1] = scy; f[2] = scz; f[3] = S.ssx; f[4] = S.ssy; f[5] = S.ssz; f[6] = sax; f[7] = say; f[8] = saz;
     }
     if (digest) {
         uint64_t* d = digest + 4 * (size_t)i;
-        d[0] = (uint64_t)nc | ((uint64_t)ns << 21) | ((uint64_t)na << 42);
-        d[1] = hc; d[2] = hs; d[3] = ha;
+        d[0] = (uint64_t)S.nc | ((uint64_t)S.ns << 21) | ((uint64_t)S.na << 42);
+        d[1] = S.hc; d[2] = S.hs; d[3] = S.ha;
     }
     integrate<float>(P, ox, oy, oz, ovx, ovy, ovz, dx, dy, dz);
     out.x[i] = ox; out.y[i] = oy; out.z[i] = oz; out.vx[i] = ovx; out.vy[i] = ovy; out.vz[i] = ovz;
@@ -214,7 +357,7 @@ hipError_t launch_f32_grid_step(const DevParams& P, StateF32 in, StateF32 out, c
                                 uint64_t* digest, hipStream_t stream)
 {
     if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_f32_grid<kF32GridBlock>, dim3(blocks_of(P.n, kF32GridBlock)), dim3(kF32GridBlock), 0,
+    hipLaunchKernelGGL(k_f32_grid_wave, dim3(blocks_of(P.n, kF32WaveOwners)), dim3(kF32WaveOwners * 64), 0,
                        stream, P, in, out, rec, vrec, key, start, forces, digest);
     return hipGetLastError();
 }

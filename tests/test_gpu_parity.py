@@ -164,6 +164,39 @@ def test_f32_grid_equals_allpairs(n, F, cells):
     say(f"float grid == float all-pairs over 153 updates, N={n}, {b.info().cells_per_axis} cells per axis")
 
 
+def test_f32_grid_crowded_owners_and_oracle():
+    # 1500 boids inside a cube of edge 0.01 (every pair within every radius):
+    # most of their owners have more in-range boids than the per-wave sorted
+    # list holds (1024) and take the serial 27-way merge; everyone else the
+    # sorted list.  Both must equal the float all-pairs step and the float
+    # oracle.
+    n, F = 6000, 1.0
+    rng = np.random.default_rng(21)
+    st = iso_state(n, F, 5)
+    sel = rng.permutation(n)[:1500]
+    st[sel, 0:3] = (0.4 + 0.01 * rng.random((1500, 3))).astype(np.float32)
+    a = make(n, F, CFG0, E.MODE_ALLPAIRS_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
+    b = make(n, F, CFG0, E.MODE_GRID_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
+    assert b.info().mode == E.MODE_GRID_F32REF
+    a.set_state(xyzv6=st)
+    b.set_state(xyzv6=st)
+    P = O.params(F, **CFG0)
+    p, v = st[:, :3].copy(), st[:, 3:].copy()
+    _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
+    # most crowded owners see more than 1024 of the others in their view cone
+    # (those heading out of the cube see fewer)
+    assert int(((d[sel, 0] & 0x1FFFFF) > 1024).sum()) > 500
+    for t in range(3):
+        a.step(1)
+        b.step(1)
+        pa, va = a.state_f32()
+        pb, vb = b.state_f32()
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb), t
+        assert np.array_equal(a.digest(), b.digest()) and np.array_equal(a.forces_f32(), b.forces_f32()), t
+        if t == 0:
+            assert np.array_equal(pb, p) and np.array_equal(vb, v) and np.array_equal(b.digest(), d)
+
+
 # ------------------------------------------------------- FP64 all-pairs ---
 def test_f64_allpairs_bit_exact_with_oracle():
     n, F = 1024, 1.0
