@@ -557,17 +557,20 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     uint4 me = make_uint4(0, 0, 0, 0);
     int L = sx;
     // Owner data that evaluating lanes read from this lane's registers: the
-    // lattice record and the FP32 velocity with 1/|v|.  A vanishing FP32
-    // velocity makes the FP32 cosine meaningless: 1/|v| = 0 marks it, and
-    // every pair of the owner is rechecked.  The FP64 state is read again
-    // for phase B instead of being held through both phases.
-    float4 fv4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // lattice record and the FP32 unit velocity.  A velocity whose FP32
+    // square vanishes or is not finite (or a NaN state) has no FP32
+    // direction: its unit vector is NaN, which makes every cosine of the
+    // owner NaN, so no cone verdict is taken in FP32 and each pair within
+    // reach of a rule is decided by the FP64 expression.  The FP64 state is
+    // read again for phase B instead of being held through both phases.
+    float3 fu = make_float3(0.f, 0.f, 0.f);
     if (active) {
         me = lat.p[s];
         L = (int)((key_sorted[s] - cbase) % Gx) - cxa + sx;
         const float fvx = (float)in.vx[s], fvy = (float)in.vy[s], fvz = (float)in.vz[s];
         const float fvv = fvx * fvx + fvy * fvy + fvz * fvz;
-        fv4 = make_float4(fvx, fvy, fvz, (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : 0.f);
+        const float rs = (fvv > 1e-30f) ? __builtin_amdgcn_rsqf(fvv) : __builtin_nanf("");
+        fu = make_float3(fvx * rs, fvy * rs, fvz * rs);
     }
     const uint2 mc = coarse_of(me.x, me.y, me.z);
     const v2s my_xy = __builtin_bit_cast(v2s, mc.x);
@@ -633,7 +636,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
     // One (owner, candidate) pair per lane: the owner's data from LDS, the
     // candidate's lattice record from L2 by slot; all reads issued ahead
     // of the evaluation.
-    struct Rec { uint4 q, m; float4 w, ov; uint32_t e, o, j; bool valid; };
+    struct Rec { uint4 q, m; float4 w; float3 ou; uint32_t e, o, j; bool valid; };
     // sorted slot of a list entry's candidate (pass base `base`): the tile
     // entry (cell) of its tile position gives the cell's slot base
     auto slot_of = [&](uint32_t ent, uint32_t base) {
@@ -655,12 +658,12 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
         // every lane of the wave is active here: each reads its owner's registers
         const uint32_t ol = r.e & 63u;
         r.m = make_uint4(lane_read(me.x, ol), lane_read(me.y, ol), lane_read(me.z, ol), lane_read(me.w, ol));
-        r.ov = make_float4(lane_read(fv4.x, ol), lane_read(fv4.y, ol), lane_read(fv4.z, ol), lane_read(fv4.w, ol));
+        r.ou = make_float3(lane_read(fu.x, ol), lane_read(fu.y, ol), lane_read(fu.z, ol));
         const int32_t ix = (int32_t)(r.q.x - r.m.x), iy = (int32_t)(r.q.y - r.m.y), iz = (int32_t)(r.q.z - r.m.z);
         const float gx = (float)ix, gy = (float)iy, gz = (float)iz;
         const float s2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
         const float inv_d = __builtin_amdgcn_rsqf(s2);
-        const float c = fmaf(r.ov.z, gz, fmaf(r.ov.y, gy, r.ov.x * gx)) * inv_d * r.ov.w;
+        const float c = fmaf(r.ou.z, gz, fmaf(r.ou.y, gy, r.ou.x * gx)) * inv_d;
         const float band = P.band_k * inv_d;
         const float c_dn = c - band, c_up = c + band;
         // padding, the owner itself, or a boid whose state is NaN (bit 31 of
@@ -684,8 +687,7 @@ __global__ __launch_bounds__(NB) BS_NBR_OCCUPANCY void k_nbr_mixed(Flocks FL, St
             rule_bands(P, SEP, s2, c, c_dn, c_up, skip, in_s, out_s);
             rule_bands(P, ALI, s2, c, c_dn, c_up, skip, in_a, out_a);
         }
-        const bool obad = !(r.ov.w > 0.f);
-        const bool sure = (in_c | out_c) & (in_s | out_s) & (in_a | out_a) & !(obad & !(out_c & out_s & out_a));
+        const bool sure = (in_c | out_c) & (in_s | out_s) & (in_a | out_a);
         if (!sure) {
             const uint32_t so = t0 + r.o;
             const double ovx = in.vx[so], ovy = in.vy[so], ovz = in.vz[so];

@@ -317,6 +317,34 @@ def test_rules_without_view_cone(mode, name):
     say(f"{name} {E.MODE_NAMES[mode]} ok")
 
 
+CONE45 = dict(sep=(0.02, 0.25, 0.002), ali=(0.05, 0.25, 0.06), coh=(0.08, 0.25, 0.008), vmax=0.005, vmin=0.001)
+
+
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F64, E.MODE_GRID_MIXED])
+def test_owner_velocity_without_fp32_direction(mode):
+    # An owner moving at 1e-20 has a well-defined FP64 direction while its
+    # FP32 squared speed underflows: the mixed kernel may take no cone verdict
+    # from its FP32 cosine.  With a 45-degree cone (a = 0.25) the boid ahead
+    # is a cohesion and alignment neighbour, those behind and beside are not.
+    F = 1.0
+    st = np.array([[0.5, 0.5, 0.5, 1e-20, 0, 0], [0.53, 0.5, 0.5, 0.003, 0, 0],
+                   [0.47, 0.5, 0.5, 0.003, 0, 0], [0.5, 0.53, 0.5, 0.003, 0, 0],
+                   [0.5, 0.5, 0.515, 0.0, 0.0, -1e-20], [0.1, 0.1, 0.1, 0.003, 0, 0]], np.float64)
+    e = make(len(st), F, CONE45, mode, E.FLAG_DIGEST)
+    assert e.info().mode == mode
+    e.set_state(pos=st[:, :3], vel=st[:, 3:])
+    p, v = st[:, :3].copy(), st[:, 3:].copy()
+    _, d = O.step(O.params(F, **CONE45), p, v, digest=True)
+    e.step(1)
+    gd = e.digest()
+    assert (d[0, 0] & 0x1FFFFF) == 1 and (d[4, 0] & 0x1FFFFF) == 1, d[:, 0]   # the oracle's own verdicts
+    assert np.array_equal(gd, d), (gd[:, 0], d[:, 0])
+    gp, gv = e.state_f64()
+    ep, ev = scaled_err(gp, gv, p, v, F, CONE45["vmax"])
+    tol = 1e-12 if mode == E.MODE_GRID_F64 else 1e-6
+    assert ep <= tol and ev <= tol, (ep, ev)
+
+
 def test_grid_f64_ten_steps_continuous():
     n, F = 16384, 1.0
     st = iso_state(n, F, 1)
