@@ -4,10 +4,12 @@
 // because each boid's candidates are visited in ascending index order
 // (:186 copies the primary cell's list; phase A of :153-212 sums in float in
 // visit order).  Here the cell list is a counting sort of the float state by
-// (cell, id); an owner visits the in-range boids of its 27 cells in ascending
-// id by a lazy 27-way merge (each cell's ids ascend; the smallest head is
-// taken next), so every float sum adds the same terms in the same order as
-// the reference.  The state stays in id order (as boidsim keeps it).
+// (cell, id); one wave per owner collects the boids of its 27 cells within
+// the largest radius, sorts them by id and adds their terms in ascending id
+// (a serial 27-way merge by id serves owners with more boids in reach than
+// the wave's list holds), so every float sum adds the same terms in the same
+// order as the reference.  The state stays in id order (as boidsim keeps
+// it).
 //
 // Compiled with -ffp-contract=off: every multiply and add is rounded on its
 // own, as the reference's x86-64 build computes it.
@@ -164,16 +166,18 @@ __device__ SumsF32 merge_serial(const DevParams& P, const float4* __restrict__ r
     return S;
 }
 
-// Wave form: one wave per owner (id order).  (1) The lanes take the
-// decisions for the candidates of the 27 cells, 64 at a time in (cell,
-// slot) order, and append the in-range ones to a per-wave LDS list as
-// (id << 32 | slot).  (2) The list is sorted by id (bitonic network in LDS,
-// one wave).  (3) The lanes compute 64 sorted pairs' terms at a time and
-// the wave adds them in ascending id, one pair at a time, every lane
-// holding the same sums: the reference's order.  An owner with more
-// in-range boids than the list holds takes the serial merge on lane 0.
-constexpr int kF32WaveOwners = 4;            // owners (waves) per workgroup
-constexpr uint32_t kF32HitCap = 1024;        // in-range boids per owner sorted in LDS
+// Wave form: one one-wave workgroup per owner (id order).  (1) The lanes
+// test the candidates of the 27 cells, 64 at a time in (cell, slot) order,
+// against the largest radius and append those within it to an LDS list as
+// (id << 32 | slot).  (2) The list is sorted by id (bitonic network in LDS).
+// (3) For 64 sorted boids at a time the lanes take the rule decisions and
+// write the pairs' nine float terms to an LDS table, 0 where a rule does
+// not hold; lane c < 9 then adds column c of the table in ascending id, so
+// each float sum adds the reference's terms in the reference's order (a
+// running sum that starts at +0 is never -0, so adding +0 leaves it
+// unchanged bit for bit).  An owner with more boids within the largest
+// radius than the list holds takes the serial merge on lane 0.
+constexpr uint32_t kF32HitCap = 1024;        // boids per owner sorted in LDS
 
 // Orders this wave's LDS writes before its later LDS reads (a wave's LDS
 // operations execute in order; this keeps the compiler from moving them).
@@ -189,18 +193,29 @@ __device__ __forceinline__ T lane_of(T v, uint32_t l)
     return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)l));
 }
 
-__global__ __launch_bounds__(kF32WaveOwners * 64) void k_f32_grid_wave(DevParams P, StateF32 in, StateF32 out,
-                                                                      const float4* __restrict__ rec,
-                                                                      const float4* __restrict__ vrec,
-                                                                      const uint32_t* __restrict__ key,
-                                                                      const uint32_t* __restrict__ start,
-                                                                      float* forces, uint64_t* digest)
+// 64-bit sum over the wave (butterfly), in every lane.
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
-    __shared__ uint64_t hits[kF32WaveOwners][kF32HitCap];
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t i = blockIdx.x * kF32WaveOwners + w;
-    if (i >= P.n) return;                                // the whole wave; no workgroup barrier below
-    uint64_t* hb = hits[w];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, StateF32 out,
+                                                      const float4* __restrict__ rec,
+                                                      const float4* __restrict__ vrec,
+                                                      const uint32_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ start, float* forces,
+                                                      uint64_t* digest)
+{
+    __shared__ uint64_t hb[kF32HitCap];
+    __shared__ float terms[9][64];                     // term c of the block's pair l: terms[c][l]
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x;                     // the grid has one workgroup per boid
     const float px = in.x[i], py = in.y[i], pz = in.z[i];
     const float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
     const float vn = norm3(vx, vy, vz);
@@ -226,26 +241,31 @@ __global__ __launch_bounds__(kF32WaveOwners * 64) void k_f32_grid_wave(DevParams
         const uint32_t t = __shfl_up(incl, d, 64);
         if (lane >= d) incl += t;
     }
-    const uint32_t excl = incl - cn;
+    const uint32_t excl = incl - cn;                   // non-decreasing over lanes 0..26
     const uint32_t M = (uint32_t)__builtin_amdgcn_readlane((int)incl, 26);
 
-    // (1) decisions, hits appended in (cell, slot) order
+    // (1) candidates within the largest radius, appended in (cell, slot) order
     uint32_t H = 0;
     for (uint32_t m0 = 0; m0 < M; m0 += 64) {
         const uint32_t m = min(m0 + lane, M - 1);
+        // the last cell k in 0..26 with excl[k] <= m (binary lifting over the lanes)
         uint32_t kk = 0;
-        for (uint32_t k = 1; k < 27; ++k)
-            kk = m >= (uint32_t)__builtin_amdgcn_readlane((int)excl, (int)k) ? k : kk;
+#pragma unroll
+        for (uint32_t st = 16; st > 0; st >>= 1) {
+            const uint32_t pr = kk + st;
+            const uint32_t ex = (uint32_t)__shfl((int)excl, (int)min(pr, 26u), 64);
+            kk = (pr <= 26u && ex <= m) ? pr : kk;
+        }
         const uint32_t slot = (uint32_t)__shfl((int)ca, (int)kk, 64) + (m - (uint32_t)__shfl((int)excl, (int)kk, 64));
         bool hit = false;
         uint32_t j = 0;
         if (m0 + lane < M) {
             const float4 q = rec[slot];
             j = __builtin_bit_cast(uint32_t, q.w);
-            if (j != i) {
-                const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
-                hit = r.in_c | r.in_s | r.in_a;
-            }
+            const float ex = px - min_image(q.x, px, P.F, P.half);
+            const float ey = py - min_image(q.y, py, P.F, P.half);
+            const float ez = pz - min_image(q.z, pz, P.F, P.half);
+            hit = (j != i) && (ex * ex + ey * ey + ez * ez < P.s_max_f);
         }
         const uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
         const uint32_t pos = H + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -256,7 +276,7 @@ __global__ __launch_bounds__(kF32WaveOwners * 64) void k_f32_grid_wave(DevParams
 
     SumsF32 S;
     if (H > kF32HitCap) {
-        // more in-range boids than the list holds: the serial merge (lane 0)
+        // more boids within reach than the list holds: the serial merge (lane 0)
         if (lane != 0) return;
         uint32_t* scratch = (uint32_t*)hb;
         S = merge_serial(P, rec, vrec, start, i, px, py, pz, vx, vy, vz, vn, cx, cy, cz, scratch, scratch + 27,
@@ -277,38 +297,50 @@ __global__ __launch_bounds__(kF32WaveOwners * 64) void k_f32_grid_wave(DevParams
                 wave_lds_order();
             }
         }
-        // (3) terms of 64 sorted pairs at a time, added in ascending id
+        // (3) terms of 64 sorted pairs at a time; lane c < 9 sums column c
+        float sum = 0.f;
+        uint64_t hc = 0, hs = 0, ha = 0;                // this lane's pairs' id hashes
         for (uint32_t h0 = 0; h0 < H; h0 += 64) {
             const uint32_t h = h0 + lane;
-            int flags = 0;
-            float qx = 0, qy = 0, qz = 0, tx = 0, ty = 0, tz = 0, wx = 0, wy = 0, wz = 0;
-            uint32_t j = 0;
+            float t[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            bool fc = false, fs = false, fa = false;
             if (h < H) {
                 const uint64_t e = hb[h];
                 const uint32_t slot = (uint32_t)e;
-                j = (uint32_t)(e >> 32);
                 const float4 q = rec[slot];
                 const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
-                flags = (r.in_c ? 1 : 0) | (r.in_s ? 2 : 0) | (r.in_a ? 4 : 0);
-                qx = r.qx; qy = r.qy; qz = r.qz;
-                if (r.in_s) {
-                    const float t = sqrtf(r.s);
-                    tx = r.ex / t; ty = r.ey / t; tz = r.ez / t;
+                fc = r.in_c; fs = r.in_s; fa = r.in_a;
+                if (fc) { t[0] = r.qx; t[1] = r.qy; t[2] = r.qz; }
+                if (fs) {
+                    const float d = sqrtf(r.s);
+                    t[3] = r.ex / d; t[4] = r.ey / d; t[5] = r.ez / d;
                 }
-                if (r.in_a) {
+                if (fa) {
                     const float4 v = vrec[slot];
-                    wx = v.x; wy = v.y; wz = v.z;
+                    t[6] = v.x; t[7] = v.y; t[8] = v.z;
+                }
+                if (digest) {
+                    const uint64_t hh = mix64((uint32_t)(e >> 32));
+                    hc += fc ? hh : 0; hs += fs ? hh : 0; ha += fa ? hh : 0;
                 }
             }
-            const uint32_t cnt = min(64u, H - h0);
-            for (uint32_t l = 0; l < cnt; ++l) {             // ascending id: the reference's order
-                const int f = __builtin_amdgcn_readlane(flags, (int)l);
-                const uint64_t hh = mix64((uint32_t)__builtin_amdgcn_readlane((int)j, (int)l));
-                if (f & 1) { S.nc++; S.scx += lane_of(qx, l); S.scy += lane_of(qy, l); S.scz += lane_of(qz, l); S.hc += hh; }
-                if (f & 2) { S.ns++; S.ssx += lane_of(tx, l); S.ssy += lane_of(ty, l); S.ssz += lane_of(tz, l); S.hs += hh; }
-                if (f & 4) { S.na++; S.sax += lane_of(wx, l); S.say += lane_of(wy, l); S.saz += lane_of(wz, l); S.ha += hh; }
+            S.nc += __popcll(__builtin_amdgcn_ballot_w64(fc));
+            S.ns += __popcll(__builtin_amdgcn_ballot_w64(fs));
+            S.na += __popcll(__builtin_amdgcn_ballot_w64(fa));
+#pragma unroll
+            for (int k = 0; k < 9; ++k) terms[k][lane] = t[k];
+            wave_lds_order();
+            if (lane < 9) {
+                const float* col = terms[lane];
+                const uint32_t cnt = min(64u, H - h0);
+                for (uint32_t l = 0; l < cnt; ++l) sum += col[l];   // ascending id: the reference's order
             }
+            wave_lds_order();
         }
+        S.scx = lane_of(sum, 0); S.scy = lane_of(sum, 1); S.scz = lane_of(sum, 2);
+        S.ssx = lane_of(sum, 3); S.ssy = lane_of(sum, 4); S.ssz = lane_of(sum, 5);
+        S.sax = lane_of(sum, 6); S.say = lane_of(sum, 7); S.saz = lane_of(sum, 8);
+        if (digest) { S.hc = wave_sum_u64(hc); S.hs = wave_sum_u64(hs); S.ha = wave_sum_u64(ha); }
         if (lane != 0) return;
     } else if (lane != 0) {
         return;
@@ -357,8 +389,8 @@ hipError_t launch_f32_grid_step(const DevParams& P, StateF32 in, StateF32 out, c
                                 uint64_t* digest, hipStream_t stream)
 {
     if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_f32_grid_wave, dim3(blocks_of(P.n, kF32WaveOwners)), dim3(kF32WaveOwners * 64), 0,
-                       stream, P, in, out, rec, vrec, key, start, forces, digest);
+    hipLaunchKernelGGL(k_f32_grid_wave, dim3(P.n), dim3(64), 0, stream, P, in, out, rec, vrec, key, start, forces,
+                       digest);
     return hipGetLastError();
 }
 
