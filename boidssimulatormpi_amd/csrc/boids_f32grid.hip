@@ -168,16 +168,17 @@ __device__ SumsF32 merge_serial(const DevParams& P, const float4* __restrict__ r
 
 // Wave form: one one-wave workgroup per owner (id order).  (1) The lanes
 // test the candidates of the 27 cells, 64 at a time in (cell, slot) order,
-// against the largest radius and append those within it to an LDS list as
-// (id << 32 | slot).  (2) The list is sorted by id (bitonic network in LDS).
-// (3) For 64 sorted boids at a time the lanes take the rule decisions and
+// against the largest radius and append the ids of those within it to an
+// LDS list.  (2) The list is sorted (bitonic network in LDS).  (3) For 64
+// sorted boids at a time the lanes read their states by id, take the rule
+// decisions and
 // write the pairs' nine float terms to an LDS table, 0 where a rule does
 // not hold; lane c < 9 then adds column c of the table in ascending id, so
 // each float sum adds the reference's terms in the reference's order (a
 // running sum that starts at +0 is never -0, so adding +0 leaves it
 // unchanged bit for bit).  An owner with more boids within the largest
 // radius than the list holds takes the serial merge on lane 0.
-constexpr uint32_t kF32HitCap = 1024;        // boids per owner sorted in LDS
+constexpr uint32_t kF32HitCap = 2048;        // boids per owner sorted in LDS
 
 // Orders this wave's LDS writes before its later LDS reads (a wave's LDS
 // operations execute in order; this keeps the compiler from moving them).
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
                                                       const uint32_t* __restrict__ start, float* forces,
                                                       uint64_t* digest)
 {
-    __shared__ uint64_t hb[kF32HitCap];
+    __shared__ uint32_t hb[kF32HitCap];
     __shared__ float terms[9][64];                     // term c of the block's pair l: terms[c][l]
     const uint32_t lane = threadIdx.x;
     const uint32_t i = blockIdx.x;                     // the grid has one workgroup per boid
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
         const uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
         const uint32_t pos = H + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        if (hit && pos < kF32HitCap) hb[pos] = ((uint64_t)j << 32) | slot;
+        if (hit && pos < kF32HitCap) hb[pos] = j;
         H += (uint32_t)__popcll(mask);
     }
 
@@ -278,20 +279,18 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
     if (H > kF32HitCap) {
         // more boids within reach than the list holds: the serial merge (lane 0)
         if (lane != 0) return;
-        uint32_t* scratch = (uint32_t*)hb;
-        S = merge_serial(P, rec, vrec, start, i, px, py, pz, vx, vy, vz, vn, cx, cy, cz, scratch, scratch + 27,
-                         scratch + 54);
+        S = merge_serial(P, rec, vrec, start, i, px, py, pz, vx, vy, vz, vn, cx, cy, cz, hb, hb + 27, hb + 54);
     } else if (H > 0) {
-        // (2) bitonic sort of the list by id, padded to a power of two
+        // (2) bitonic sort of the ids, padded to a power of two
         uint32_t P2 = 1;
         while (P2 < H) P2 <<= 1;
-        for (uint32_t t = H + lane; t < P2; t += 64) hb[t] = ~0ull;
+        for (uint32_t t = H + lane; t < P2; t += 64) hb[t] = ~0u;
         wave_lds_order();
         for (uint32_t size = 2; size <= P2; size <<= 1) {
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                 for (uint32_t t = lane; t < P2 / 2; t += 64) {
                     const uint32_t a = 2 * t - (t & (stride - 1)), b = a + stride;
-                    const uint64_t ka = hb[a], kb = hb[b];
+                    const uint32_t ka = hb[a], kb = hb[b];
                     if ((ka > kb) == ((a & size) == 0)) { hb[a] = kb; hb[b] = ka; }
                 }
                 wave_lds_order();
@@ -305,22 +304,17 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
             float t[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             bool fc = false, fs = false, fa = false;
             if (h < H) {
-                const uint64_t e = hb[h];
-                const uint32_t slot = (uint32_t)e;
-                const float4 q = rec[slot];
-                const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, q.x, q.y, q.z);
+                const uint32_t j = hb[h];
+                const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, in.x[j], in.y[j], in.z[j]);
                 fc = r.in_c; fs = r.in_s; fa = r.in_a;
                 if (fc) { t[0] = r.qx; t[1] = r.qy; t[2] = r.qz; }
                 if (fs) {
                     const float d = sqrtf(r.s);
                     t[3] = r.ex / d; t[4] = r.ey / d; t[5] = r.ez / d;
                 }
-                if (fa) {
-                    const float4 v = vrec[slot];
-                    t[6] = v.x; t[7] = v.y; t[8] = v.z;
-                }
+                if (fa) { t[6] = in.vx[j]; t[7] = in.vy[j]; t[8] = in.vz[j]; }
                 if (digest) {
-                    const uint64_t hh = mix64((uint32_t)(e >> 32));
+                    const uint64_t hh = mix64(j);
                     hc += fc ? hh : 0; hs += fs ? hh : 0; ha += fa ? hh : 0;
                 }
             }

@@ -165,16 +165,20 @@ def test_f32_grid_equals_allpairs(n, F, cells):
 
 
 def test_f32_grid_crowded_owners_and_oracle():
-    # 1500 boids inside a cube of edge 0.01 (every pair within every radius):
-    # most of their owners have more in-range boids than the per-wave sorted
-    # list holds (1024) and take the serial 27-way merge; everyone else the
-    # sorted list.  Both must equal the float all-pairs step and the float
-    # oracle.
-    n, F = 6000, 1.0
+    # Two crowded cubes of edge 0.01 (every pair inside a cube lies within
+    # every radius).  Cube A holds 2500 boids: each of their owners has 2499
+    # boids within the largest radius, more than the per-wave sorted list
+    # holds (2048), and takes the serial 27-way merge.  Cube B holds 1500:
+    # their owners sort lists of 1500+ ids (2048 with the padding).  Everyone
+    # else sorts short lists.  All must equal the float all-pairs step and the
+    # float oracle.
+    n, F = 8000, 1.0
     rng = np.random.default_rng(21)
     st = iso_state(n, F, 5)
-    sel = rng.permutation(n)[:1500]
-    st[sel, 0:3] = (0.4 + 0.01 * rng.random((1500, 3))).astype(np.float32)
+    perm = rng.permutation(n)
+    sel_a, sel_b = perm[:2500], perm[2500:4000]
+    st[sel_a, 0:3] = (0.4 + 0.01 * rng.random((2500, 3))).astype(np.float32)
+    st[sel_b, 0:3] = (0.7 + 0.01 * rng.random((1500, 3))).astype(np.float32)
     a = make(n, F, CFG0, E.MODE_ALLPAIRS_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
     b = make(n, F, CFG0, E.MODE_GRID_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
     assert b.info().mode == E.MODE_GRID_F32REF
@@ -183,9 +187,10 @@ def test_f32_grid_crowded_owners_and_oracle():
     P = O.params(F, **CFG0)
     p, v = st[:, :3].copy(), st[:, 3:].copy()
     _, d = O.step(P, p, v, digest=True, enumeration=O.ENUM_CELLS)
-    # most crowded owners see more than 1024 of the others in their view cone
-    # (those heading out of the cube see fewer)
-    assert int(((d[sel, 0] & 0x1FFFFF) > 1024).sum()) > 500
+    # the cubes are as crowded as intended: cohesion counts (view cone
+    # included) above 2048 in A and above 1024 in B
+    assert int(((d[sel_a, 0] & 0x1FFFFF) > 2048).sum()) > 100
+    assert int(((d[sel_b, 0] & 0x1FFFFF) > 1024).sum()) > 500
     for t in range(3):
         a.step(1)
         b.step(1)
