@@ -178,7 +178,7 @@ __device__ SumsF32 merge_serial(const DevParams& P, const float4* __restrict__ r
 // running sum that starts at +0 is never -0, so adding +0 leaves it
 // unchanged bit for bit).  An owner with more boids within the largest
 // radius than the list holds takes the serial merge on lane 0.
-constexpr uint32_t kF32HitCap = 2048;        // boids per owner sorted in LDS
+constexpr uint32_t kF32HitCap = 1024;        // boids per owner sorted in LDS
 
 // Orders this wave's LDS writes before its later LDS reads (a wave's LDS
 // operations execute in order; this keeps the compiler from moving them).
@@ -245,9 +245,9 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
     const uint32_t excl = incl - cn;                   // non-decreasing over lanes 0..26
     const uint32_t M = (uint32_t)__builtin_amdgcn_readlane((int)incl, 26);
 
-    // (1) candidates within the largest radius, appended in (cell, slot) order
-    uint32_t H = 0;
-    for (uint32_t m0 = 0; m0 < M; m0 += 64) {
+    // (1) candidates within the largest radius, appended in (cell, slot) order;
+    // each batch's records are fetched while the previous batch is tested
+    auto record_of = [&](uint32_t m0) {
         const uint32_t m = min(m0 + lane, M - 1);
         // the last cell k in 0..26 with excl[k] <= m (binary lifting over the lanes)
         uint32_t kk = 0;
@@ -258,16 +258,18 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
             kk = (pr <= 26u && ex <= m) ? pr : kk;
         }
         const uint32_t slot = (uint32_t)__shfl((int)ca, (int)kk, 64) + (m - (uint32_t)__shfl((int)excl, (int)kk, 64));
-        bool hit = false;
-        uint32_t j = 0;
-        if (m0 + lane < M) {
-            const float4 q = rec[slot];
-            j = __builtin_bit_cast(uint32_t, q.w);
-            const float ex = px - min_image(q.x, px, P.F, P.half);
-            const float ey = py - min_image(q.y, py, P.F, P.half);
-            const float ez = pz - min_image(q.z, pz, P.F, P.half);
-            hit = (j != i) && (ex * ex + ey * ey + ez * ez < P.s_max_f);
-        }
+        return rec[slot];
+    };
+    uint32_t H = 0;
+    float4 qn = M ? record_of(0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t m0 = 0; m0 < M; m0 += 64) {
+        const float4 q = qn;
+        if (m0 + 64 < M) qn = record_of(m0 + 64);
+        const uint32_t j = __builtin_bit_cast(uint32_t, q.w);
+        const float ex = px - min_image(q.x, px, P.F, P.half);
+        const float ey = py - min_image(q.y, py, P.F, P.half);
+        const float ez = pz - min_image(q.z, pz, P.F, P.half);
+        const bool hit = (m0 + lane < M) && (j != i) && (ex * ex + ey * ey + ez * ez < P.s_max_f);
         const uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
         const uint32_t pos = H + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
