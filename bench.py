@@ -29,9 +29,10 @@ over the timed updates (`traffic` and `valu` are read from the committed
 rocprofv3 summaries); `cpu_baseline` times the CPU path beside it on this
 host for a bounded sample, with all host threads and with one.
 
-Other workloads (--workload): flock16k (N=16384, F=1, T=8000, FP64 plain,
-mixed and the float grid mode that reproduces the reference's bytes; the
-CPU beside it is the reference's own grid engine built from source,
+Other workloads (--workload): flock16k (N=16384, F=1, T=8000; headline:
+the float grid mode that reproduces the reference's bytes, also FP64 plain
+and mixed; the CPU beside it is the reference's own grid engine built from
+source,
 oracle/_ref/ref_grid), flock1k (N=1024, F=1, T=100, all-pairs FP64 and the
 reference-faithful float mode; CPU: the reference's own all-pairs engine,
 oracle/_ref/ref_allpairs) and blobs16m (N=2^24, F=10, 64 clustered blobs,
@@ -70,7 +71,7 @@ WORKLOADS = {
                     modes="grid_f64,grid_mixed", head="grid_mixed", cpu="port", sweep=True,
                     what="uniform positions, isotropic velocities"),
     "flock16k": dict(n=16384, field=1.0, seed=1, init=E.INIT_ISOTROPIC, steps=8000,
-                     modes="grid_f64,grid_mixed,grid_f32ref", head="grid_f64", cpu="ref_grid", sweep=False,
+                     modes="grid_f32ref,grid_f64,grid_mixed", head="grid_f32ref", cpu="ref_grid", sweep=False,
                      what="uniform positions, isotropic velocities"),
     "flock1k": dict(n=1024, field=1.0, seed=0, init=E.INIT_ISOTROPIC, steps=100,
                     modes="allpairs_f64,allpairs_f32ref", head="allpairs_f64", cpu="ref_allpairs", sweep=False,
@@ -407,8 +408,9 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="flock1m", choices=sorted(WORKLOADS),
-                    help="flock1m: the headline (N=2^20, F=4); flock16k: N=16384, F=1, FP64 plain vs the "
-                         "reference's own grid engine on the CPU; flock1k: N=1024 all-pairs vs the reference's "
+                    help="flock1m: the headline (N=2^20, F=4); flock16k: N=16384, F=1, the float grid mode "
+                         "(also FP64 plain and mixed) vs the reference's own grid engine on the CPU; flock1k: "
+                         "N=1024 all-pairs vs the reference's "
                          "own all-pairs engine; blobs16m: N=2^24, F=10, 64 clustered blobs")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed updates (default: the whole run); fewer than the run's length are stratified "
@@ -472,7 +474,7 @@ def main():
             f"boid-steps/s (from t=0 only: {res[name]['from_t0']['ms_per_step']:.3f} ms/step)")
 
     head = res.get(wl["head"]) or next(iter(res.values()))
-    bytes_per = BYTES_PER_BOID_STEP // 2 if head["mode"] == "allpairs_f32ref" else BYTES_PER_BOID_STEP
+    bytes_per = BYTES_PER_BOID_STEP // 2 if head["mode"] in ("allpairs_f32ref", "grid_f32ref") else BYTES_PER_BOID_STEP
     achieved = bytes_per * n / (head["pair_us_avg"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(head["mode"], n),
@@ -498,7 +500,7 @@ def main():
         timed = (f"{a.steps} single updates stratified over the {T_run}-update run from t = {a.from_t} "
                  f"(t = {at}: midpoints of {a.steps} equal segments), each bracketed by device syncs, "
                  f"the rest untimed")
-    dtype = {"grid_mixed": "f32+f64", "allpairs_f32ref": "f32"}.get(head["mode"], "f64")
+    dtype = {"grid_mixed": "f32+f64", "allpairs_f32ref": "f32", "grid_f32ref": "f32"}.get(head["mode"], "f64")
     out = {
         "metric": metric,
         "value": head["boid_steps_per_s"],
