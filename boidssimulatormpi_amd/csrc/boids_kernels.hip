@@ -60,14 +60,25 @@ __global__ __launch_bounds__(kBlock) void k_allpairs_wave(DevParams P, St in, St
     int nc = 0, ns = 0, na = 0;
     uint64_t hc = 0, hs = 0, ha = 0;
 
+    // candidate positions two batches ahead of the one being tested
+    auto fetch = [&](uint32_t base, R& x, R& y, R& z) {
+        const uint32_t j = min(base + lane, n - 1);
+        x = in.x[j]; y = in.y[j]; z = in.z[j];
+    };
+    R ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    fetch(0, ax, ay, az);
+    if (64 < n) fetch(64, bx, by, bz);
     for (uint32_t base = 0; base < n; base += 64) {
+        const R cx = ax, cy = ay, cz = az;
+        ax = bx; ay = by; az = bz;
+        if (base + 128 < n) fetch(base + 128, bx, by, bz);
         const uint32_t j = base + lane;
         bool in_c = false, in_s = false, in_a = false;
         R qx = 0, qy = 0, qz = 0, tx = 0, ty = 0, tz = 0, wx = 0, wy = 0, wz = 0;
         if (j < n && j != i) {
-            qx = min_image(in.x[j], px, P.F, P.half);
-            qy = min_image(in.y[j], py, P.F, P.half);
-            qz = min_image(in.z[j], pz, P.F, P.half);
+            qx = min_image(cx, px, P.F, P.half);
+            qy = min_image(cy, py, P.F, P.half);
+            qz = min_image(cz, pz, P.F, P.half);
             const R ex = px - qx, ey = py - qy, ez = pz - qz;
             const R s = ex * ex + ey * ey + ez * ez;
             if (s < smax) {
