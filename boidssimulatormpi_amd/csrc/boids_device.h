@@ -13,6 +13,17 @@ namespace boids {
 
 typedef short v2s __attribute__((ext_vector_type(2)));   // packed int16 pair
 
+// Ids below `me` in the ascending run ids[0, len).
+__device__ __forceinline__ uint32_t count_below(const uint32_t* ids, uint32_t len, uint32_t me)
+{
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ids[mid] < me) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t j)
 {
     uint64_t z = j + 0x9E3779B97F4A7C15ull;

@@ -79,7 +79,9 @@ __global__ __launch_bounds__(kBlock) void k_f32_scatter(uint32_t n, const uint32
 
 // Final slot = cell start + the id's rank among the cell's ids, so ids
 // ascend inside every cell; the slot's record is the candidate position
-// with the id in .w, and the (unshifted) velocity.
+// with the id in .w, and the (unshifted) velocity.  perm's runs of 64 slots
+// are sorted by (cell, id) (launch_sort_runs): the rank is the slot's place
+// in its own part of the cell plus a binary search in each other part.
 __global__ __launch_bounds__(kBlock) void k_f32_cells(uint32_t n, StateF32 st, const uint32_t* __restrict__ key,
                                                       const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ perm, float4* rec, float4* vrec)
@@ -89,8 +91,13 @@ __global__ __launch_bounds__(kBlock) void k_f32_cells(uint32_t n, StateF32 st, c
     const uint32_t me = perm[t];
     const uint32_t c = key[me];
     const uint32_t lo = start[c], hi = start[c + 1];
-    uint32_t r = 0;
-    for (uint32_t u = lo; u < hi; ++u) r += perm[u] < me ? 1u : 0u;
+    const uint32_t run = t >> 6;
+    uint32_t r = t - max(lo, run << 6);
+    for (uint32_t v = lo >> 6; (v << 6) < hi; ++v) {
+        if (v == run) continue;
+        const uint32_t a = max(lo, v << 6), b = min(hi, (v + 1) << 6);
+        r += count_below(perm + a, b - a, me);
+    }
     const uint32_t f = lo + r;
     rec[f] = make_float4(st.x[me], st.y[me], st.z[me], __builtin_bit_cast(float, me));
     vrec[f] = make_float4(st.vx[me], st.vy[me], st.vz[me], 0.f);
@@ -375,6 +382,8 @@ hipError_t launch_f32_grid_sort(const DevParams& P, StateF32 st, uint32_t* key, 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_f32_scatter, dim3(blocks_of(P.n, kBlock)), dim3(kBlock), 0, stream, P.n,
                        (const uint32_t*)key, (const uint32_t*)rank, (const uint32_t*)start, perm);
+    e = launch_sort_runs(P.n, key, perm, perm, stream);     // ids in id order: index = id
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_f32_cells, dim3(blocks_of(P.n, kBlock)), dim3(kBlock), 0, stream, P.n, st,
                        (const uint32_t*)key, (const uint32_t*)start, (const uint32_t*)perm, rec, vrec);
     return hipGetLastError();

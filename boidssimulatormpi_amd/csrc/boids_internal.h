@@ -128,6 +128,10 @@ hipError_t launch_scatter(uint32_t n, const uint32_t* key, const uint32_t* start
 // sorted parts into its in-cell id order).
 hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
                                  const uint32_t* id, uint32_t* perm, uint32_t* permid, hipStream_t stream);
+// The run sort alone: every run of 64 slots of perm / permid (boid index /
+// id, grouped by cell key[perm]) sorted by (cell, id) in place.  perm and
+// permid may be one array where a boid's index is its id.
+hipError_t launch_sort_runs(uint32_t n, const uint32_t* key, uint32_t* perm, uint32_t* permid, hipStream_t stream);
 // lat (mixed) receives the lattice copy; rec (FP64 grid) the records the
 // neighbour kernel fetches, three planes of n double2: (x, y), (z, vx), (vy, vz).
 hipError_t launch_gather(Flocks F, const uint32_t* perm, const uint32_t* permid, const uint32_t* key,

@@ -361,17 +361,6 @@ __global__ __launch_bounds__(kBlock) void k_sort_runs(uint32_t n, const uint32_t
     }
 }
 
-// Ids below `me` in the ascending run ids[0, len).
-__device__ __forceinline__ uint32_t count_below(const uint32_t* ids, uint32_t len, uint32_t me)
-{
-    uint32_t lo = 0, hi = len;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ids[mid] < me) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 // In-cell order, step 2: final slot = cell start + rank of the boid id among
 // its cell's ids, so the order inside a cell is ascending id whatever the
 // atomics did.  The rank is the slot's position in its own sorted part plus
@@ -514,6 +503,12 @@ hipError_t launch_scatter_ranked(uint32_t n, const uint32_t* key, const uint32_t
                        perm, permid);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
+    return launch_sort_runs(n, key, perm, permid, stream);
+}
+
+hipError_t launch_sort_runs(uint32_t n, const uint32_t* key, uint32_t* perm, uint32_t* permid, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
     if (n <= (1u << 26))
         hipLaunchKernelGGL(k_sort_runs<uint32_t>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, n, key, perm, permid);
     else
