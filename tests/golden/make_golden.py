@@ -9,6 +9,7 @@ committed and travel without the reference.
 
     python tests/golden/make_golden.py               # every fixture
     python tests/golden/make_golden.py grid_testini  # only ref_grid_testini_n1000.npz
+    python tests/golden/make_golden.py grid_cfg1     # only ref_grid_cfg1_n16384.npz
 """
 from __future__ import annotations
 
@@ -76,10 +77,31 @@ def grid_testini():
          vptcl_sha256=np.frombuffer(hashlib.sha256(ptcl_bytes(vel, n, T, F)).digest(), np.uint8))
 
 
+def grid_cfg1():
+    """7. boidsim_grid at full cfg1 size: N = 16384, F = 1, cfg0 rules, the
+    reference's random_uniform init after srand48(1), 200 updates.  Stored as
+    digests only (the fixture stays small): SHA-256 of the initial state and
+    of the .ptcl files of positions and velocities (states 0..199)."""
+    n, T, F = 16384, 200, 1.0
+    d = run_ref("ref_grid", n, T, F, CFG0, "random_uniform", 1, list(range(T + 1)))
+    pos = {t: d[t][0] for t in d}
+    vel = {t: d[t][1] for t in d}
+    init = np.concatenate([d[0][0], d[0][1]], 1).astype("<f4")
+    save("ref_grid_cfg1_n16384.npz", rules=rules_array(F, CFG0), steps=np.int64(T),
+         init_sha256=np.frombuffer(hashlib.sha256(init.tobytes()).digest(), np.uint8),
+         ptcl_sha256=np.frombuffer(hashlib.sha256(ptcl_bytes(pos, n, T, F)).digest(), np.uint8),
+         vptcl_sha256=np.frombuffer(hashlib.sha256(ptcl_bytes(vel, n, T, F)).digest(), np.uint8),
+         final_sha256=np.frombuffer(hashlib.sha256(
+             np.concatenate([d[T][0], d[T][1]], 1).astype("<f4").tobytes()).digest(), np.uint8))
+
+
 def main():
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile.ref")], check=True)
     if sys.argv[1:] == ["grid_testini"]:      # one fixture only (the others stay byte-identical)
         grid_testini()
+        return
+    if sys.argv[1:] == ["grid_cfg1"]:
+        grid_cfg1()
         return
 
     # 1. boidsim, random_uniform init (drand48 never seeded), cfg0 rules
@@ -133,6 +155,7 @@ def main():
          pos0=d[0][0], vel0=d[0][1], pos20=d[20][0], vel20=d[20][1])
 
     grid_testini()
+    grid_cfg1()
 
 
 if __name__ == "__main__":

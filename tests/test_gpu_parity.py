@@ -126,6 +126,31 @@ def test_f32_grid_matches_reference_grid(golden_dir, name, steps):
     assert np.array_equal(p, g[f"pos{steps}"]) and np.array_equal(v, g[f"vel{steps}"])
 
 
+@pytest.mark.parametrize("mode", [E.MODE_GRID_F32REF, E.MODE_ALLPAIRS_F32REF])
+def test_f32_cfg1_full_size_matches_reference_ptcl(golden_dir, mode):
+    # boidsim_grid at the full cfg1 size (N = 16384, F = 1, its random_uniform
+    # init after srand48(1)): the reference grid engine's .ptcl digests of
+    # positions and velocities over 200 updates, byte for byte (boidsim's
+    # all-pairs arithmetic gives the same bytes)
+    g = np.load(os.path.join(golden_dir, "ref_grid_cfg1_n16384.npz"))
+    F, rules = rules_of(g["rules"])
+    n, T = 16384, int(g["steps"])
+    st = B.generate_init(E.INIT_RANDOM_UNIFORM, n, F, rules["vmax"], rules["vmin"], seed=1)
+    assert hashlib.sha256(st.astype("<f4").tobytes()).digest() == g["init_sha256"].tobytes()
+    e = make(n, F, rules, mode)
+    assert e.info().mode == mode
+    e.set_state(xyzv6=st)
+    hdr = struct.pack("<4sBBIIIIIffffff", b"PTCL", 0, 2, 50, n, T, 0, 30, 0.0, F, 0.0, F, 0.0, F)
+    hp, hv = hashlib.sha256(hdr), hashlib.sha256(hdr)
+    for _ in range(T):
+        p, v = e.state_f32()
+        hp.update(p.astype("<f4").tobytes()); hv.update(v.astype("<f4").tobytes())
+        e.step(1)
+    p, v = e.state_f32()
+    assert hp.digest() == g["ptcl_sha256"].tobytes() and hv.digest() == g["vptcl_sha256"].tobytes()
+    assert hashlib.sha256(np.concatenate([p, v], 1).astype("<f4").tobytes()).digest() == g["final_sha256"].tobytes()
+
+
 def test_f32_grid_file_init_forces(golden_dir):
     g = np.load(os.path.join(golden_dir, "ref_allpairs_file_n1024.npz"))
     F, rules = rules_of(g["rules"])

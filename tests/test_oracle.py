@@ -146,6 +146,24 @@ def test_f32_grid_testini_ptcl(golden_dir, enum):
     assert np.array_equal(p, g["pos30"]) and np.array_equal(v, g["vel30"])
 
 
+def test_f32_grid_cfg1_full_size_ptcl(golden_dir):
+    # boidsim_grid at the full cfg1 size (N = 16384, F = 1, cfg0 rules, its
+    # random_uniform init after srand48(1)): the oracle's cell enumeration
+    # reproduces the reference grid engine's .ptcl digests over 200 updates
+    g = load(golden_dir, "ref_grid_cfg1_n16384.npz")
+    P = params_from(g["rules"])
+    n, T = 16384, int(g["steps"])
+    p, v = O.init_reference("random_uniform", n, 1.0, 0.005, 0.001, seed=1)
+    assert hashlib.sha256(np.concatenate([p, v], 1).astype("<f4").tobytes()).digest() == g["init_sha256"].tobytes()
+    hp = hashlib.sha256(ptcl([], n, T, 1.0))          # the header alone
+    hv = hashlib.sha256(ptcl([], n, T, 1.0))
+    for _ in range(T):
+        hp.update(p.astype("<f4").tobytes()); hv.update(v.astype("<f4").tobytes())
+        O.step(P, p, v, enumeration=O.ENUM_CELLS)
+    assert hp.digest() == g["ptcl_sha256"].tobytes() and hv.digest() == g["vptcl_sha256"].tobytes()
+    assert hashlib.sha256(np.concatenate([p, v], 1).astype("<f4").tobytes()).digest() == g["final_sha256"].tobytes()
+
+
 # --------------------------------------------------- FP64: closed forms ---
 def f64(pos, vel):
     return np.array(pos, np.float64).reshape(-1, 3), np.array(vel, np.float64).reshape(-1, 3)
