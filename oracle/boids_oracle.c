@@ -221,7 +221,7 @@ int bo_metrics(uint32_t n, double F, const double* pos, const double* vel, doubl
     if (!out || (n && (!pos || !vel))) return -1;
     memset(out, 0, sizeof(*out));
     if (n == 0) return 0;
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    omp_set_num_threads(nthreads > 0 ? nthreads : omp_get_num_procs());   /* 0: every host thread */
     double sx = 0, sy = 0, sz = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const double* v = vel + 3 * (size_t)i;

@@ -104,7 +104,8 @@ def _ptr(a, ct):
 def step(p: Params, pos, vel, *, forces=False, digest=False, enumeration=ENUM_ALLPAIRS,
          nthreads=0):
     """One update() in place on pos/vel (float32 -> reference-faithful f32
-    instantiation, float64 -> f64 instantiation).  Returns (forces, digest)."""
+    instantiation, float64 -> f64 instantiation).  Returns (forces, digest).
+    nthreads: OpenMP threads (0: every host thread)."""
     assert pos.dtype == vel.dtype and pos.flags.c_contiguous and vel.flags.c_contiguous
     n = pos.shape[0]
     assert pos.shape == (n, 3) and vel.shape == (n, 3)
