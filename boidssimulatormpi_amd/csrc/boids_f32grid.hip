@@ -178,9 +178,9 @@ __device__ SumsF32 merge_serial(const DevParams& P, const float4* __restrict__ r
 // against the largest radius and append the ids of those within it to an
 // LDS list.  (2) The list is sorted (bitonic network in LDS).  (3) For 64
 // sorted boids at a time the lanes read their states by id, take the rule
-// decisions and
-// write the pairs' nine float terms to an LDS table, 0 where a rule does
-// not hold; lane c < 9 then adds column c of the table in ascending id, so
+// decisions and write the pairs' nine float terms to an LDS table, 0 where
+// a rule does not hold; lane c < 9 then adds column c of the table in
+// ascending id, so
 // each float sum adds the reference's terms in the reference's order (a
 // running sum that starts at +0 is never -0, so adding +0 leaves it
 // unchanged bit for bit).  An owner with more boids within the largest
