@@ -290,30 +290,14 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
         if (lane != 0) return;
         S = merge_serial(P, rec, vrec, start, i, px, py, pz, vx, vy, vz, vn, cx, cy, cz, hb, hb + 27, hb + 54);
     } else if (H > 0) {
-        // (2) bitonic sort of the ids, padded to a power of two
-        uint32_t P2 = 1;
-        while (P2 < H) P2 <<= 1;
-        for (uint32_t t = H + lane; t < P2; t += 64) hb[t] = ~0u;
-        wave_lds_order();
-        for (uint32_t size = 2; size <= P2; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t t = lane; t < P2 / 2; t += 64) {
-                    const uint32_t a = 2 * t - (t & (stride - 1)), b = a + stride;
-                    const uint32_t ka = hb[a], kb = hb[b];
-                    if ((ka > kb) == ((a & size) == 0)) { hb[a] = kb; hb[b] = ka; }
-                }
-                wave_lds_order();
-            }
-        }
-        // (3) terms of 64 sorted pairs at a time; lane c < 9 sums column c
+        // (3), one block: the terms of the sorted boids h0 .. h0 + 63 (lane
+        // l holds the id of boid h0 + l in j); lane c < 9 sums column c
         float sum = 0.f;
         uint64_t hc = 0, hs = 0, ha = 0;                // this lane's pairs' id hashes
-        for (uint32_t h0 = 0; h0 < H; h0 += 64) {
-            const uint32_t h = h0 + lane;
+        auto block = [&](uint32_t h0, uint32_t j) __attribute__((always_inline)) {
             float t[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             bool fc = false, fs = false, fa = false;
-            if (h < H) {
-                const uint32_t j = hb[h];
+            if (h0 + lane < H) {
                 const PairF32 r = decide_f32(P, px, py, pz, vx, vy, vz, vn, in.x[j], in.y[j], in.z[j]);
                 fc = r.in_c; fs = r.in_s; fa = r.in_a;
                 if (fc) { t[0] = r.qx; t[1] = r.qy; t[2] = r.qz; }
@@ -339,6 +323,64 @@ __global__ __launch_bounds__(64) void k_f32_grid_wave(DevParams P, StateF32 in, 
                 for (uint32_t l = 0; l < cnt; ++l) sum += col[l];   // ascending id: the reference's order
             }
             wave_lds_order();
+        };
+        if (H <= 4 * 64) {
+            // (2) bitonic sort in registers: element e = 64 r + lane in
+            // register r, at least 64 elements, padded with ~0u
+            constexpr uint32_t R = 4;
+            uint32_t P2 = 64;
+            while (P2 < H) P2 <<= 1;
+            uint32_t kr[R];
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) kr[r] = 64 * r + lane < H ? hb[64 * r + lane] : ~0u;
+            for (uint32_t size = 2; size <= P2; size <<= 1) {
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    if (stride >= 64) {                      // partner in register r ^ (stride / 64), same lane
+                        auto cx = [&](uint32_t& a, uint32_t& b, uint32_t r) {
+                            const bool asc = ((64 * r + lane) & size) == 0;
+                            const uint32_t lo = min(a, b), hi = max(a, b);
+                            a = asc ? lo : hi;
+                            b = asc ? hi : lo;
+                        };
+                        if (stride == 64) {
+                            cx(kr[0], kr[1], 0);
+                            if (P2 > 128) cx(kr[2], kr[3], 2);
+                        } else {                             // stride 128: P2 = 256
+                            cx(kr[0], kr[2], 0);
+                            cx(kr[1], kr[3], 1);
+                        }
+                    } else {                                 // partner in lane ^ stride, same register
+#pragma unroll
+                        for (uint32_t r = 0; r < R; ++r) {
+                            if (64 * r >= P2) continue;
+                            const uint32_t pk = (uint32_t)__shfl_xor((int)kr[r], (int)stride, 64);
+                            const bool keep_min = ((lane & stride) == 0) == (((64 * r + lane) & size) == 0);
+                            kr[r] = keep_min ? min(kr[r], pk) : max(kr[r], pk);
+                        }
+                    }
+                }
+            }
+            block(0, kr[0]);
+            if (H > 64) block(64, kr[1]);
+            if (H > 128) block(128, kr[2]);
+            if (H > 192) block(192, kr[3]);
+        } else {
+            // (2) bitonic sort of the ids in LDS, padded to a power of two
+            uint32_t P2 = 1;
+            while (P2 < H) P2 <<= 1;
+            for (uint32_t t = H + lane; t < P2; t += 64) hb[t] = ~0u;
+            wave_lds_order();
+            for (uint32_t size = 2; size <= P2; size <<= 1) {
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (uint32_t t = lane; t < P2 / 2; t += 64) {
+                        const uint32_t a = 2 * t - (t & (stride - 1)), b = a + stride;
+                        const uint32_t ka = hb[a], kb = hb[b];
+                        if ((ka > kb) == ((a & size) == 0)) { hb[a] = kb; hb[b] = ka; }
+                    }
+                    wave_lds_order();
+                }
+            }
+            for (uint32_t h0 = 0; h0 < H; h0 += 64) block(h0, h0 + lane < H ? hb[h0 + lane] : 0u);
         }
         S.scx = lane_of(sum, 0); S.scy = lane_of(sum, 1); S.scz = lane_of(sum, 2);
         S.ssx = lane_of(sum, 3); S.ssy = lane_of(sum, 4); S.ssz = lane_of(sum, 5);
