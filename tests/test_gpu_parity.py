@@ -166,11 +166,14 @@ def test_f32_grid_file_init_forces(golden_dir):
             assert np.array_equal(e.forces_f32() * wvec(rules), g[f"force{t}"]), t
 
 
-@pytest.mark.parametrize("n,F,cells", [(16384, 1.0, 0), (3000, 0.25, 3), (20000, 1.6, 0)])
+@pytest.mark.parametrize("n,F,cells", [(16384, 1.0, 0), (3000, 0.25, 3), (20000, 1.6, 0), (16384, 0.62, 0)])
 def test_f32_grid_equals_allpairs(n, F, cells):
     # the float grid equals the float all-pairs step bit for bit over a long
     # run (chaotic: any difference in a sum's order would grow), with its
-    # neighbour digests and forces; also on the smallest grid (3 cells per axis)
+    # neighbour digests and forces; also on the smallest grid (3 cells per
+    # axis).  Boids within the largest radius per owner: ~35 at N = 16384,
+    # F = 1 (sorted in one register), ~150 at F = 0.62 (two and four
+    # registers), ~400 at N = 3000, F = 0.25 (sorted in LDS)
     st = iso_state(n, F, 3)
     a = make(n, F, CFG0, E.MODE_ALLPAIRS_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES)
     b = make(n, F, CFG0, E.MODE_GRID_F32REF, E.FLAG_DIGEST | E.FLAG_FORCES, cells=cells)
